@@ -1,0 +1,12 @@
+"""celllistmap.jl_amd -- MI355X (gfx950) cell-list pairwise engine with the
+ParticleSystem / pairwise! / neighborlist / InPlaceNeighborList interface of
+CellListMap.jl.  See core.py for the API mapping; the device code is csrc/ and
+its C ABI is include/clm_hip.h at the repository root."""
+from ._lib import LibraryNotBuilt, build_library, lib  # noqa: F401
+from .core import (  # noqa: F401
+    ArgumentError, CapacityError, DimensionMismatch, DistanceHistogram, EnergyForces, InPlaceNeighborList,
+    MinimumDistance, NeighborListOp, NeighborListResult, PairwiseVelocities, ParticleSystem, SoftCoreEnergy,
+    SoftCoreEnergyForces, VelocityHistogram, box_info, EnergyGradients, energy_adjoint, neighborlist, neighborlist_, pairwise, pairwise_, update_,
+)
+
+__version__ = "0.1.0"

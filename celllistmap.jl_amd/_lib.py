@@ -1,0 +1,102 @@
+"""ctypes binding of libclm_hip.so (the C ABI declared in include/clm_hip.h).
+
+The library is built in-tree by ``build_library()`` (hipcc, gfx950).  Loading it is
+mandatory: every device operator of this package goes through it, and a missing or
+unloadable library raises ``LibraryNotBuilt`` -- there is no CPU fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libclm_hip.so")
+CSRC = os.path.join(HERE, "csrc")
+
+CLM_OK, CLM_ERR_ARG, CLM_ERR_DIM, CLM_ERR_CELL, CLM_ERR_NAN = 0, 1, 2, 3, 4
+CLM_ERR_CAPACITY, CLM_ERR_HIP, CLM_ERR_STATE, CLM_ERR_RANGE = 5, 6, 7, 8
+CLM_F32, CLM_F64 = 0, 1
+ORTHORHOMBIC, TRICLINIC, NONPERIODIC = 0, 1, 2
+
+EXPORTS = [
+    "clm_create", "clm_destroy", "clm_last_error", "clm_set_stream", "clm_version",
+    "clm_set_box", "clm_get_box", "clm_box_compute",
+    "clm_set_positions", "clm_set_types", "clm_set_velocities",
+    "clm_update_cell_list", "clm_cell_list_info",
+    "clm_neighborlist", "clm_energy_forces", "clm_energy_adjoint", "clm_histogram", "clm_min_distance",
+    "clm_profile_enable", "clm_profile_get", "clm_profile_reset",
+]
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+class BoxInfo(C.Structure):
+    _fields_ = [("dim", C.c_int), ("celltype", C.c_int), ("lcell", C.c_int), ("nc", C.c_int * 3),
+                ("cutoff", C.c_double), ("cutoff_sqr", C.c_double), ("input", C.c_double * 9),
+                ("aligned", C.c_double * 9), ("rotation", C.c_double * 9), ("inv_rotation", C.c_double * 9),
+                ("cell_size", C.c_double * 3), ("cbmin", C.c_double * 3), ("cbmax", C.c_double * 3),
+                ("origin", C.c_double * 3)]
+
+
+class SoftCore(C.Structure):
+    _fields_ = [("ntypes", C.c_int), ("eps", C.c_double * 8), ("sigma", C.c_double * 8)]
+
+
+def build_library(verbose: bool = False) -> str:
+    """Compile csrc/ into libclm_hip.so with hipcc for gfx950 (always rebuilds)."""
+    cmd = ["make", "-B", "-C", CSRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(r.stdout)
+    return LIBPATH
+
+
+_LIB = None
+
+
+def lib():
+    """Load libclm_hip.so (raises LibraryNotBuilt when it is missing)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise LibraryNotBuilt(f"{LIBPATH} is missing: run __graft_entry__.build() (hipcc) first")
+    try:
+        L = C.CDLL(LIBPATH)
+    except OSError as e:  # pragma: no cover - depends on the ROCm runtime
+        raise LibraryNotBuilt(f"cannot load {LIBPATH}: {e}") from e
+    P, I, I64, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    sig = {
+        "clm_create": ([C.POINTER(P), I, I, I], I),
+        "clm_destroy": ([P], I),
+        "clm_last_error": ([P], C.c_char_p),
+        "clm_set_stream": ([P, P], I),
+        "clm_version": ([], C.c_char_p),
+        "clm_set_box": ([P, P, I, D, I], I),
+        "clm_get_box": ([P, C.POINTER(BoxInfo)], I),
+        "clm_box_compute": ([I, I, P, I, D, I, P, C.POINTER(BoxInfo)], I),
+        "clm_set_positions": ([P, P, I64, I], I),
+        "clm_set_types": ([P, P, I64, I], I),
+        "clm_set_velocities": ([P, P, I64, I], I),
+        "clm_update_cell_list": ([P], I),
+        "clm_cell_list_info": ([P, P], I),
+        "clm_neighborlist": ([P, P, P, P, I64, C.POINTER(I64), I], I),
+        "clm_energy_forces": ([P, C.POINTER(SoftCore), C.POINTER(D), P, I, C.POINTER(I64)], I),
+        "clm_energy_adjoint": ([P, C.POINTER(SoftCore), C.POINTER(D), P, P, P, I], I),
+        "clm_histogram": ([P, I, D, P, P], I),
+        "clm_min_distance": ([P, C.POINTER(I64), C.POINTER(I64), C.POINTER(D)], I),
+        "clm_profile_enable": ([P, I], I),
+        "clm_profile_get": ([P, C.c_char_p, C.POINTER(D), C.POINTER(I64)], I),
+        "clm_profile_reset": ([P], I),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _LIB = L
+    return L

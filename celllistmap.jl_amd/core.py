@@ -1,0 +1,648 @@
+"""Host-side mirror of CellListMap.jl's API for the device cell-list engine.
+
+Reference API (CellListMap.jl src/API/*.jl) -> this module:
+
+    ParticleSystem(; xpositions, unitcell, cutoff, output, lcell, ...)   ParticleSystem(...)
+    pairwise!(f, sys; reset)                                           pairwise_(op, sys, reset=True)
+    update!(sys; xpositions, cutoff, unitcell)                         update_(sys, ...)
+    neighborlist(; xpositions, cutoff, unitcell, lcell)                neighborlist(...)
+    InPlaceNeighborList(; ...), neighborlist!(nl), update!(nl; ...)    InPlaceNeighborList(...), neighborlist_(nl), update_(nl, ...)
+    resize_output!(sys, n)                                             (device buffers grow on demand)
+
+A Julia closure cannot run on the GPU, so ``pairwise_`` takes one of the fixed
+device operators defined below (soft-core energy/forces, neighbour list,
+distance histogram with radial velocities, minimum distance).  Indices are
+0-based.  Positions are (N, D) arrays -- numpy on the host or torch tensors on
+the GPU -- with the memory layout of Julia's Vector{SVector{D,T}}.
+
+Errors mirror the reference: ArgumentError for invalid cells, cutoffs, lcell and
+NaN coordinates (Box.jl:192-194,243; CellOperations.jl:6-21); DimensionMismatch for
+mismatched dimensions (CellLists.jl:740-751).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import BoxInfo, SoftCore
+
+__all__ = [
+    "ArgumentError", "DimensionMismatch", "CapacityError",
+    "ParticleSystem", "pairwise_", "pairwise", "update_",
+    "neighborlist", "neighborlist_", "InPlaceNeighborList",
+    "SoftCoreEnergyForces", "SoftCoreEnergy", "NeighborListOp", "DistanceHistogram",
+    "PairwiseVelocities", "MinimumDistance", "NeighborListResult", "box_info", "energy_adjoint",
+    "EnergyGradients",
+]
+
+
+class ArgumentError(ValueError):
+    """Counterpart of Julia's ArgumentError."""
+
+
+class DimensionMismatch(ValueError):
+    """Counterpart of Julia's DimensionMismatch."""
+
+
+class CapacityError(RuntimeError):
+    pass
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_torch(x) -> bool:
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+def _raise(ctx, code: int, what: str = ""):
+    if code == _lib.CLM_OK:
+        return
+    msg = _lib.lib().clm_last_error(ctx).decode() if ctx else ""
+    text = f"{what}: {msg}" if what else msg
+    if code in (_lib.CLM_ERR_ARG, _lib.CLM_ERR_CELL, _lib.CLM_ERR_NAN, _lib.CLM_ERR_RANGE):
+        raise ArgumentError(text)
+    if code == _lib.CLM_ERR_DIM:
+        raise DimensionMismatch(text)
+    if code == _lib.CLM_ERR_CAPACITY:
+        raise CapacityError(text)
+    raise RuntimeError(f"clm error {code}: {text}")
+
+
+def _dtype_code(dt) -> int:
+    if dt in (np.float32, "float32", "f32"):
+        return _lib.CLM_F32
+    if dt in (np.float64, "float64", "f64"):
+        return _lib.CLM_F64
+    name = str(dt)
+    if "float32" in name:
+        return _lib.CLM_F32
+    if "float64" in name:
+        return _lib.CLM_F64
+    raise ArgumentError(f"unsupported dtype {dt}")
+
+
+def _np_dtype(code: int):
+    return np.float32 if code == _lib.CLM_F32 else np.float64
+
+
+def _unitcell(unitcell, dim: int):
+    """(celltype, column-major cell) from a vector of sides, a matrix or None."""
+    if unitcell is None:
+        return _lib.NONPERIODIC, None
+    uc = np.asarray(unitcell, dtype=np.float64)
+    if uc.ndim == 1:
+        if uc.shape[0] != dim:
+            raise DimensionMismatch(f"unit cell has dimension {uc.shape[0]}, positions have {dim}")
+        return _lib.ORTHORHOMBIC, np.ascontiguousarray(np.diag(uc).T.reshape(-1))
+    if uc.ndim == 2 and uc.shape[0] == uc.shape[1]:
+        if uc.shape[0] != dim:
+            raise DimensionMismatch(f"unit cell has dimension {uc.shape[0]}, positions have {dim}")
+        return _lib.TRICLINIC, np.ascontiguousarray(uc.T.reshape(-1))  # column-major
+    raise ArgumentError("Unit cell matrix must be square.")
+
+
+def box_info(unitcell, cutoff, lcell: int = 1, dtype=np.float64, dim: Optional[int] = None) -> dict:
+    """Box(unitcell, cutoff; lcell) geometry computed by the library's host code
+    (no GPU needed): nc, cell sizes, computing box, aligned cell and rotation."""
+    uc = np.asarray(unitcell, dtype=np.float64)
+    d = uc.shape[0] if dim is None else dim
+    ct, cell = _unitcell(uc, d)
+    out = BoxInfo()
+    rc = _lib.lib().clm_box_compute(_dtype_code(dtype), d, cell.ctypes.data_as(C.c_void_p), ct, float(cutoff),
+                                    int(lcell), None, C.byref(out))
+    if rc:
+        if rc == _lib.CLM_ERR_CELL:
+            raise ArgumentError(" Unit cell matrix does not satisfy required conditions.")
+        if rc == _lib.CLM_ERR_DIM:
+            raise DimensionMismatch("dimension must be 2 or 3")
+        raise ArgumentError("lcell must be greater or equal to 1")
+    return _box_dict(out)
+
+
+def _box_dict(b: BoxInfo) -> dict:
+    d = b.dim
+    m = lambda a: np.array(a[:]).reshape(3, 3, order="F")[:d, :d]  # noqa: E731
+    return dict(dim=d, celltype=b.celltype, lcell=b.lcell, nc=list(b.nc[:d]), cutoff=b.cutoff,
+                cutoff_sqr=b.cutoff_sqr, cell_size=list(b.cell_size[:d]), cbmin=list(b.cbmin[:d]),
+                cbmax=list(b.cbmax[:d]), origin=list(b.origin[:d]), input=m(b.input), aligned=m(b.aligned),
+                rotation=m(b.rotation), inv_rotation=m(b.inv_rotation))
+
+
+class _Context:
+    """Owns one clm_ctx (device state of one particle system)."""
+
+    def __init__(self, dtype_code: int, dim: int, device: int = 0):
+        self.lib = _lib.lib()
+        self.ptr = C.c_void_p()
+        rc = self.lib.clm_create(C.byref(self.ptr), int(device), int(dtype_code), int(dim))
+        if rc:
+            if rc == _lib.CLM_ERR_DIM:
+                raise DimensionMismatch("dimension must be 2 or 3")
+            raise RuntimeError(f"clm_create failed with code {rc} (is a GPU visible?)")
+        self.dtype_code = dtype_code
+        self.dim = dim
+        self.device = device
+
+    def close(self):
+        if self.ptr:
+            self.lib.clm_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover
+            pass
+
+    def check(self, code, what=""):
+        _raise(self.ptr, code, what)
+
+    def use_torch_stream(self):
+        torch = _torch()
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        self.check(self.lib.clm_set_stream(self.ptr, C.c_void_p(s)), "set_stream")
+
+
+def _positions(x, dtype_code: Optional[int], dim: Optional[int] = None):
+    """Normalise positions: returns (array_or_tensor, on_device, n, dim, dtype_code)."""
+    if _is_torch(x):
+        t = x
+        if t.dim() == 1 and t.numel() == 0:
+            t = t.reshape(0, dim or 3)
+        if t.dim() != 2:
+            raise ArgumentError(" Coordinates must be vectors of vectors, or matrices.")
+        code = dtype_code if dtype_code is not None else _dtype_code(t.dtype)
+        want = _torch().float32 if code == _lib.CLM_F32 else _torch().float64
+        t = t.to(dtype=want).contiguous()
+        return t, bool(t.is_cuda), t.shape[0], t.shape[1], code
+    a = np.asarray(x)
+    if a.ndim == 1 and a.size == 0:
+        a = a.reshape(0, dim or 3)
+    if a.ndim != 2:
+        raise ArgumentError(" Coordinates must be vectors of vectors, or matrices.")
+    if dtype_code is None:
+        dtype_code = _lib.CLM_F32 if a.dtype == np.float32 else _lib.CLM_F64
+    a = np.ascontiguousarray(a, dtype=_np_dtype(dtype_code))
+    return a, False, a.shape[0], a.shape[1], dtype_code
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if _is_torch(a):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# device operators (the fixed closures)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SoftCoreEnergyForces:
+    """u(d) = eps_ij [ (sigma_ij^2/(d^2+1))^6 - (sigma_ij^2/(d^2+1))^3 ] summed over pairs
+    within the cutoff; forces F_i = -dE/dx_i.  Scalars eps/sigma, or per-type arrays with
+    Lorentz-Berthelot mixing and ``types`` per particle."""
+    eps: object = 0.2
+    sigma: object = 3.0
+    types: object = None
+    forces: bool = True
+
+    def params(self) -> SoftCore:
+        p = SoftCore()
+        eps = np.atleast_1d(np.asarray(self.eps, np.float64))
+        sig = np.atleast_1d(np.asarray(self.sigma, np.float64))
+        if eps.shape != sig.shape or eps.shape[0] > 8:
+            raise ArgumentError("eps and sigma must have the same length (at most 8 types)")
+        typed = self.types is not None
+        p.ntypes = eps.shape[0] if typed else 0
+        for k in range(eps.shape[0]):
+            p.eps[k] = eps[k]
+            p.sigma[k] = sig[k]
+        return p
+
+
+@dataclass
+class SoftCoreEnergy(SoftCoreEnergyForces):
+    forces: bool = False
+
+
+@dataclass
+class NeighborListOp:
+    """push_pair! (internals/neighborlist.jl:67-76): list of (i, j, d)."""
+    capacity: Optional[int] = None
+
+
+@dataclass
+class DistanceHistogram:
+    """Counts of pairs per distance bin, bin = floor(d / width) (last bin holds d == cutoff)."""
+    nbins: int = 10
+    width: float = 1.0
+
+
+@dataclass
+class PairwiseVelocities:
+    """Counts and mean radial velocity dot(v_i - v_j, x_i - x_j)/d per distance bin."""
+    nbins: int = 10
+    width: float = 1.0
+    velocities: object = None
+
+
+@dataclass
+class MinimumDistance:
+    """(i, j, d) of the closest pair within the cutoff (i < j, ties by smallest (i, j))."""
+
+
+@dataclass
+class EnergyForces:
+    energy: float
+    forces: object
+    npairs: Optional[int] = None
+
+
+@dataclass
+class VelocityHistogram:
+    counts: np.ndarray
+    vsum: np.ndarray
+
+    @property
+    def mean(self):
+        m = np.zeros_like(self.vsum)
+        nz = self.counts > 0
+        m[nz] = self.vsum[nz] / self.counts[nz]
+        return m
+
+
+class NeighborListResult:
+    """Neighbour list (i, j, d) arrays; iterates as tuples like the reference's
+    Vector{Tuple{Int,Int,T}} (indices 0-based here)."""
+
+    def __init__(self, i, j, d):
+        self.i, self.j, self.d = i, j, d
+
+    def __len__(self):
+        return int(self.i.shape[0])
+
+    def __iter__(self):
+        for a, b, c in zip(np.asarray(self.i.cpu() if _is_torch(self.i) else self.i).tolist(),
+                           np.asarray(self.j.cpu() if _is_torch(self.j) else self.j).tolist(),
+                           np.asarray(self.d.cpu() if _is_torch(self.d) else self.d).tolist()):
+            yield (a, b, c)
+
+    def tolist(self):
+        return list(iter(self))
+
+    def canonical(self):
+        """Sorted (min(i,j), max(i,j), d) as numpy arrays (order-free comparison)."""
+        i = np.asarray(self.i.cpu() if _is_torch(self.i) else self.i).astype(np.int64)
+        j = np.asarray(self.j.cpu() if _is_torch(self.j) else self.j).astype(np.int64)
+        d = np.asarray(self.d.cpu() if _is_torch(self.d) else self.d)
+        a, b = np.minimum(i, j), np.maximum(i, j)
+        o = np.lexsort((b, a))
+        return a[o], b[o], d[o]
+
+
+# ---------------------------------------------------------------------------
+# ParticleSystem
+# ---------------------------------------------------------------------------
+
+class ParticleSystem:
+    """ParticleSystem(; xpositions, unitcell, cutoff, output, output_name, lcell)
+    (src/API/ParticleSystem.jl:142-202).  ``output`` is a device operator; the
+    result of the last ``pairwise_`` is available as ``sys.output`` (and under
+    ``output_name``)."""
+
+    def __init__(self, xpositions=None, *, positions=None, ypositions=None, unitcell=None, cutoff,
+                 output=None, output_name: str = "output", parallel: bool = True, nbatches=(0, 0),
+                 lcell: int = 1, validate_coordinates=True, dtype=None, device: int = 0):
+        if (positions is None) == (xpositions is None):
+            raise ArgumentError("Either `positions` OR `xpositions` must be defined.")
+        if ypositions is not None:
+            raise NotImplementedError("two-set (xpositions, ypositions) systems are not supported by the device engine yet")
+        x = xpositions if xpositions is not None else positions
+        code = None if dtype is None else _dtype_code(dtype)
+        xs, on_dev, n, dim, code = _positions(x, code, np.asarray(unitcell).shape[0] if unitcell is not None else None)
+        if dim not in (2, 3):
+            raise ArgumentError(f"\n    Dimension must be 2 or 3.\n    Got: x: {dim}\n")
+        if lcell < 1:
+            raise ArgumentError("lcell must be greater or equal to 1")
+        self._ctx = _Context(code, dim, device)
+        self.dim = dim
+        self.dtype = _np_dtype(code)
+        self.lcell = int(lcell)
+        self.parallel = parallel
+        self.nbatches = nbatches
+        self.validate_coordinates = validate_coordinates
+        self.output_op = output
+        self.output_name = output_name
+        self.output = None
+        self._unitcell = None if unitcell is None else np.asarray(unitcell, dtype=np.float64)
+        self._cutoff = float(cutoff)
+        self._n = n
+        self._apply_box()
+        self._set_positions(xs, on_dev, n)
+
+    # -- properties ------------------------------------------------------
+    @property
+    def cutoff(self):
+        return self._cutoff
+
+    @cutoff.setter
+    def cutoff(self, v):
+        update_(self, cutoff=v)
+
+    @property
+    def unitcell(self):
+        info = self.box
+        m = info["input"]
+        return m
+
+    @unitcell.setter
+    def unitcell(self, v):
+        update_(self, unitcell=v)
+
+    @property
+    def box(self) -> dict:
+        out = BoxInfo()
+        self._ctx.check(self._ctx.lib.clm_get_box(self._ctx.ptr, C.byref(out)), "get_box")
+        return _box_dict(out)
+
+    @property
+    def n(self):
+        return self._n
+
+    def __getattr__(self, name):
+        if name != "output_name" and name == self.__dict__.get("output_name"):
+            return self.__dict__.get("output")
+        raise AttributeError(name)
+
+    def cell_list_info(self) -> dict:
+        out = np.zeros(4, np.int64)
+        self._ctx.check(self._ctx.lib.clm_cell_list_info(self._ctx.ptr, out.ctypes.data_as(C.c_void_p)), "cell list")
+        return dict(n_particles=int(out[0]), n_cells_with_real_particles=int(out[1]),
+                    n_cells_with_particles=int(out[2]), number_of_cells=int(out[3]))
+
+    # -- internals -------------------------------------------------------
+    def _apply_box(self):
+        ct, cell = _unitcell(self._unitcell, self.dim)
+        if not (self._cutoff > 0):
+            raise ArgumentError("cutoff must be positive")
+        rc = self._ctx.lib.clm_set_box(self._ctx.ptr, cell.ctypes.data_as(C.c_void_p) if cell is not None else None,
+                                       ct, self._cutoff, self.lcell)
+        if rc == _lib.CLM_ERR_CELL:
+            raise ArgumentError(" Unit cell matrix does not satisfy required conditions.")
+        self._ctx.check(rc, "Box")
+        self.celltype = ct
+
+    def _set_positions(self, xs, on_dev, n):
+        if on_dev:
+            self._ctx.use_torch_stream()
+        self._ctx.check(self._ctx.lib.clm_set_positions(self._ctx.ptr, _ptr(xs), int(n), int(on_dev)), "positions")
+        self._n = n
+        self._xref = xs  # keep alive until the copy is done
+
+    def update_cell_list(self):
+        """UpdateCellList! (CellLists.jl:727-927) on the device."""
+        self._ctx.check(self._ctx.lib.clm_update_cell_list(self._ctx.ptr), "UpdateCellList!")
+
+
+def update_(sys, *, xpositions=None, positions=None, ypositions=None, cutoff=None, unitcell=None, parallel=None):
+    """update!(sys; xpositions, cutoff, unitcell, parallel) (src/API/updating.jl:165-187)."""
+    if isinstance(sys, InPlaceNeighborList):
+        update_(sys.sys, xpositions=xpositions, positions=positions, ypositions=ypositions, cutoff=cutoff,
+                unitcell=unitcell, parallel=parallel)
+        return sys
+    if positions is not None and xpositions is not None:
+        raise ArgumentError("Either `positions` OR `xpositions` must be provided, not both.")
+    if ypositions is not None:
+        raise ArgumentError("ypositions can only be set for a two-set particle system")
+    x = xpositions if xpositions is not None else positions
+    if x is not None:
+        xs, on_dev, n, dim, _ = _positions(x, sys._ctx.dtype_code, sys.dim)
+        if dim != sys.dim:
+            raise DimensionMismatch(f"\n Positions have dimension {dim}, but the unit cell has dimension {sys.dim}.\n")
+        sys._set_positions(xs, on_dev, n)
+    if cutoff is not None:
+        sys._cutoff = float(cutoff)
+    if unitcell is not None:
+        if sys.celltype == _lib.NONPERIODIC:
+            raise ArgumentError("\n    Manual updating of the unit cell of non-periodic systems is not allowed.\n")
+        uc = np.asarray(unitcell, dtype=np.float64)
+        ct, _ = _unitcell(uc, sys.dim)
+        if ct != sys.celltype:
+            raise ArgumentError("the unit cell type (orthorhombic/triclinic) cannot change")
+        sys._unitcell = uc
+    if cutoff is not None or unitcell is not None:
+        sys._apply_box()
+    if parallel is not None:
+        sys.parallel = parallel
+    return sys
+
+
+# ---------------------------------------------------------------------------
+# pairwise!
+# ---------------------------------------------------------------------------
+
+def _out_array(like_device: bool, shape, dtype, device):
+    if like_device:
+        torch = _torch()
+        tdt = torch.float32 if dtype == np.float32 else torch.float64
+        return torch.empty(shape, dtype=tdt, device=f"cuda:{device}")
+    return np.empty(shape, dtype=dtype)
+
+
+def pairwise_(op, sys: ParticleSystem, *, reset: bool = True, show_progress: bool = False, device_output: bool = False):
+    """pairwise!(f, sys) (src/API/pairwise.jl:48-63) with a fixed device operator ``op``.
+    The lazy cell-list rebuild of UpdateParticleSystem! happens inside the library.
+    reset=False accumulates into the previous result (numeric outputs)."""
+    if op is None:
+        op = sys.output_op
+    ctx = sys._ctx
+    L = ctx.lib
+    if device_output:
+        ctx.use_torch_stream()
+    if isinstance(op, SoftCoreEnergyForces):
+        p = op.params()
+        _set_types(sys, op.types)
+        e = C.c_double(0.0)
+        npairs = C.c_int64(0)
+        f = None
+        if op.forces:
+            f = _out_array(device_output, (sys.n, sys.dim), sys.dtype, ctx.device)
+        ctx.check(L.clm_energy_forces(ctx.ptr, C.byref(p), C.byref(e), _ptr(f), int(device_output), C.byref(npairs)),
+                  "pairwise!")
+        res = EnergyForces(e.value, f)
+        if not reset and isinstance(sys.output, EnergyForces):
+            res.energy += sys.output.energy
+            if f is not None and sys.output.forces is not None:
+                res.forces = res.forces + sys.output.forces
+        sys.output = res
+        return res
+    if isinstance(op, NeighborListOp):
+        res = _neighborlist(sys, op.capacity, device_output)
+        sys.output = res
+        return res
+    if isinstance(op, PairwiseVelocities):
+        v = op.velocities
+        if v is None:
+            raise ArgumentError("PairwiseVelocities needs velocities")
+        vs, on_dev, n, dim, _ = _positions(v, ctx.dtype_code, sys.dim)
+        if n != sys.n or dim != sys.dim:
+            raise DimensionMismatch("velocities must be (N, dim)")
+        if on_dev:
+            ctx.use_torch_stream()
+        ctx.check(L.clm_set_velocities(ctx.ptr, _ptr(vs), int(n), int(on_dev)), "velocities")
+        counts = np.zeros(op.nbins, np.int64)
+        vsum = np.zeros(op.nbins, np.float64)
+        ctx.check(L.clm_histogram(ctx.ptr, int(op.nbins), float(op.width), _ptr(counts), _ptr(vsum)), "pairwise!")
+        res = VelocityHistogram(counts, vsum)
+        if not reset and isinstance(sys.output, VelocityHistogram):
+            res = VelocityHistogram(res.counts + sys.output.counts, res.vsum + sys.output.vsum)
+        sys.output = res
+        return res
+    if isinstance(op, DistanceHistogram):
+        counts = np.zeros(op.nbins, np.int64)
+        ctx.check(L.clm_histogram(ctx.ptr, int(op.nbins), float(op.width), _ptr(counts), None), "pairwise!")
+        if not reset and isinstance(sys.output, np.ndarray) and sys.output.shape == counts.shape:
+            counts = counts + sys.output
+        sys.output = counts
+        return counts
+    if isinstance(op, MinimumDistance):
+        i, j, d = C.c_int64(0), C.c_int64(0), C.c_double(0.0)
+        ctx.check(L.clm_min_distance(ctx.ptr, C.byref(i), C.byref(j), C.byref(d)), "pairwise!")
+        res = (i.value, j.value, d.value)
+        if not reset and isinstance(sys.output, tuple) and sys.output[2] <= res[2]:
+            res = sys.output
+        sys.output = res
+        return res
+    if callable(op):
+        raise NotImplementedError("arbitrary Python closures do not run on the device; use a device operator")
+    raise ArgumentError(f"unknown operator {op!r}")
+
+
+pairwise = pairwise_
+
+
+@dataclass
+class EnergyGradients:
+    energy: float
+    dedx: object
+    deps: np.ndarray
+    dsigma: np.ndarray
+
+
+def _set_types(sys, types):
+    ctx = sys._ctx
+    if types is None:
+        return 0
+    if _is_torch(types):
+        tt = types.to(dtype=_torch().int32).contiguous()
+        on_dev = bool(tt.is_cuda)
+    else:
+        tt = np.ascontiguousarray(types, dtype=np.int32)
+        on_dev = False
+    if tt.shape[0] != sys.n:
+        raise DimensionMismatch("types must have one entry per particle")
+    ctx.check(ctx.lib.clm_set_types(ctx.ptr, _ptr(tt), int(tt.shape[0]), int(on_dev)), "types")
+    return 1
+
+
+def energy_adjoint(sys: ParticleSystem, eps=0.2, sigma=3.0, types=None, device_output: bool = False) -> EnergyGradients:
+    """Soft-core energy and its adjoint in one device sweep: dE/dx (N, dim) and dE/d(eps, sigma)
+    per type (a single entry when untyped).  Positions are re-read from the cell list; no pair
+    data is stored between the forward and the adjoint."""
+    op = SoftCoreEnergyForces(eps, sigma, types)
+    p = op.params()
+    _set_types(sys, types)
+    ctx = sys._ctx
+    if device_output:
+        ctx.use_torch_stream()
+    nt = max(p.ntypes, 1)
+    e = C.c_double(0.0)
+    dedx = _out_array(device_output, (sys.n, sys.dim), sys.dtype, ctx.device)
+    de = np.zeros(nt)
+    ds = np.zeros(nt)
+    ctx.check(ctx.lib.clm_energy_adjoint(ctx.ptr, C.byref(p), C.byref(e), _ptr(dedx), _ptr(de), _ptr(ds),
+                                         int(device_output)), "energy_adjoint")
+    return EnergyGradients(e.value, dedx, de, ds)
+
+
+def _neighborlist(sys: ParticleSystem, capacity, device_output: bool):
+    """neighborlist! body: grow-and-retry on CLM_ERR_CAPACITY (resize_output!,
+    internals/neighborlist.jl:43-64)."""
+    ctx = sys._ctx
+    L = ctx.lib
+    if capacity is None:
+        capacity = getattr(sys, "_nl_capacity", None)
+    if capacity is None:
+        # _estimated_n_pairs (internals/neighborlist.jl:43-48) with 10 % headroom
+        info = sys.box
+        vol = abs(np.linalg.det(info["input"])) if sys.celltype != _lib.NONPERIODIC else np.prod(
+            np.array(info["cbmax"]) - np.array(info["cbmin"]))
+        sph = (np.pi if sys.dim == 2 else 4 * np.pi / 3) * sys.cutoff ** sys.dim
+        est = sys.n * (sys.n - 1) / 2 * sph / max(vol, 1e-300)
+        capacity = int(est * 1.1) + 1024
+    while True:
+        if device_output:
+            torch = _torch()
+            i = torch.empty(capacity, dtype=torch.int32, device=f"cuda:{ctx.device}")
+            j = torch.empty(capacity, dtype=torch.int32, device=f"cuda:{ctx.device}")
+        else:
+            i = np.empty(capacity, np.int32)
+            j = np.empty(capacity, np.int32)
+        d = _out_array(device_output, (capacity,), sys.dtype, ctx.device)
+        npairs = C.c_int64(0)
+        rc = L.clm_neighborlist(ctx.ptr, _ptr(i), _ptr(j), _ptr(d), int(capacity), C.byref(npairs), int(device_output))
+        if rc == _lib.CLM_ERR_CAPACITY:
+            capacity = int(npairs.value * 1.1) + 1024
+            continue
+        ctx.check(rc, "neighborlist!")
+        n = npairs.value
+        sys._nl_capacity = capacity
+        return NeighborListResult(i[:n], j[:n], d[:n])
+
+
+# ---------------------------------------------------------------------------
+# neighbour lists
+# ---------------------------------------------------------------------------
+
+class InPlaceNeighborList:
+    """InPlaceNeighborList(; xpositions, cutoff, unitcell, lcell) (src/API/neighborlist.jl:84-111):
+    device buffers (cell list and pair arrays) are reused across ``neighborlist_`` calls."""
+
+    def __init__(self, xpositions=None, *, positions=None, ypositions=None, cutoff, unitcell=None,
+                 parallel: bool = True, nbatches=(0, 0), lcell: int = 1, validate_coordinates=True,
+                 show_progress: bool = False, dtype=None, device: int = 0, device_output: bool = False):
+        self.sys = ParticleSystem(xpositions, positions=positions, ypositions=ypositions, unitcell=unitcell,
+                                  cutoff=cutoff, output=NeighborListOp(), output_name="nb", parallel=parallel,
+                                  nbatches=nbatches, lcell=lcell, validate_coordinates=validate_coordinates,
+                                  dtype=dtype, device=device)
+        self.show_progress = show_progress
+        self.device_output = device_output
+
+
+def neighborlist_(system: InPlaceNeighborList):
+    """neighborlist!(system) (src/API/neighborlist.jl:217-231)."""
+    return pairwise_(NeighborListOp(), system.sys, device_output=system.device_output)
+
+
+def neighborlist(xpositions=None, *, positions=None, ypositions=None, cutoff, unitcell=None, parallel: bool = True,
+                 nbatches=(0, 0), lcell: int = 1, validate_coordinates=True, show_progress: bool = False,
+                 dtype=None, device: int = 0):
+    """neighborlist(; xpositions, cutoff, unitcell, lcell) (src/API/neighborlist.jl:314-340)."""
+    nl = InPlaceNeighborList(xpositions, positions=positions, ypositions=ypositions, cutoff=cutoff, unitcell=unitcell,
+                             parallel=parallel, nbatches=nbatches, lcell=lcell,
+                             validate_coordinates=validate_coordinates, show_progress=show_progress, dtype=dtype,
+                             device=device)
+    return neighborlist_(nl)

@@ -1,0 +1,888 @@
+// clm_api.hip -- C ABI (include/clm_hip.h) of the MI355X cell-list engine.
+//
+// One context owns a HIP stream, the particle data and grow-only device buffers.
+// The pipeline per cell-list update is
+//   k_bin_count -> exclusive scan of cell counts -> k_bin_scatter
+// and per operator call
+//   (k_home_counts -> scan -> k_row_tasks -> scan, cached per sweep mode) -> k_sweep<Op>.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "clm_box.hpp"
+#include "clm_kernels.hpp"
+#include "clm_hip.h"
+
+using namespace clm;
+
+namespace {
+
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct clm_ctx {
+    int device = 0, dtype = CLM_F32, dim = 3;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    int num_cus = 256;
+
+    // geometry
+    bool has_box = false;
+    int celltype = CLM_ORTHORHOMBIC, lcell = 1;
+    double cell_in[9] = {0};
+    double cutoff_in = 0;
+    HostBox<float> hbf;
+    HostBox<double> hbd;
+    bool box_built = false;
+
+    // particles
+    int64_t n = 0;
+    bool has_pos = false, dirty = true;
+    DBuf pos, types, vel;
+    bool has_types = false, has_vel = false;
+
+    // cell list
+    bool cl_valid = false;
+    int ncells = 0, nrows = 0;
+    int64_t n_entries = 0;
+    DBuf xw, home_cell, cnt, creal, cstart, cur_real, cur_ghost, E, flags;
+    int64_t e_cap = 0;
+    DBuf hc, ntask, hstart[3], tstart[3];
+    bool mode_valid[3] = {false, false, false};
+    DBuf counters, scan_tmp;
+
+    // operator buffers
+    DBuf nl_i, nl_j, nl_d, nl_count, outbuf, acc, hist_c, hist_v, md_d2, md_a, md_b;
+    uint32_t *h_flags = nullptr;
+
+    // measurement
+    bool prof = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, std::pair<double, int64_t>> prof_acc;
+};
+
+namespace {
+
+clm_status fail(clm_ctx *c, clm_status s, const std::string &msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+clm_status hip_fail(clm_ctx *c, hipError_t e, const char *where) {
+    return fail(c, CLM_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr)                                      \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
+    } while (0)
+
+hipError_t ensure(DBuf &b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return hipSuccess;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e == hipSuccess) b.bytes = want;
+    return e;
+}
+
+hipEvent_t take_event(clm_ctx *c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+void collect_profile(clm_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto &acc = c->prof_acc[p.name];
+            acc.first += ms;
+            acc.second += 1;
+        }
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+template <typename F>
+hipError_t timed(clm_ctx *c, const char *name, F &&launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->prof) {
+        a = take_event(c);
+        b = take_event(c);
+        hipEventRecord(a, c->stream);
+    }
+    launch();
+    hipError_t e = hipGetLastError();
+    if (c->prof) {
+        hipEventRecord(b, c->stream);
+        c->pending.push_back({name, a, b});
+    }
+    return e;
+}
+
+#define LAUNCH(c, name, ...)                                            \
+    do {                                                                \
+        hipError_t e_ = timed(c, name, [&] { __VA_ARGS__; });           \
+        if (e_ != hipSuccess) return hip_fail(c, e_, name);             \
+    } while (0)
+
+clm_status sync(clm_ctx *c) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
+    if (c->prof) collect_profile(c);
+    return CLM_OK;
+}
+
+inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+clm_status fill_int(clm_ctx *c, void *p, int64_t n, int v) {
+    if (n <= 0) return CLM_OK;
+    LAUNCH(c, "k_fill_int", hipLaunchKernelGGL(k_fill_int, dim3(blocks(n, 256)), dim3(256), 0, c->stream, (int *)p, (int)n, v));
+    return CLM_OK;
+}
+
+clm_status scan_int(clm_ctx *c, const int *in, int *out, int n) {
+    size_t need = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, n, c->stream);
+    HIPCHK(c, ensure(c->scan_tmp, need));
+    size_t have = c->scan_tmp.bytes;
+    LAUNCH(c, "scan", hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, have, in, out, n, c->stream));
+    return CLM_OK;
+}
+
+template <typename T>
+HostBox<T> &hbox(clm_ctx *c);
+template <>
+HostBox<float> &hbox<float>(clm_ctx *c) { return c->hbf; }
+template <>
+HostBox<double> &hbox<double>(clm_ctx *c) { return c->hbd; }
+
+template <typename T>
+DevBox<T> dev_box(const HostBox<T> &hb) {
+    DevBox<T> b;
+    std::memset(&b, 0, sizeof(b));
+    for (int k = 0; k < 9; ++k) { b.input[k] = hb.input.a[k]; b.rot[k] = hb.rot.a[k]; }
+    const int D = hb.dim;
+    const int nimg = D == 3 ? 27 : 9;
+    for (int m = 0; m < nimg; ++m) {
+        T kv[3] = {T(m % 3 - 1), T((m / 3) % 3 - 1), D == 3 ? T(m / 9 - 1) : T(0)};
+        for (int i = 0; i < D; ++i) {
+            // aligned * k in the order of translation_image (CellOperations.jl:138-139)
+            T s = hb.aligned(i, 0) * kv[0];
+            for (int j = 1; j < D; ++j) s = s + hb.aligned(i, j) * kv[j];
+            b.shift[m][i] = s;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        b.cbmin[k] = hb.cbmin[k];
+        b.cbmax[k] = hb.cbmax[k];
+        b.cs[k] = hb.cs[k];
+        b.nc[k] = hb.nc[k];
+    }
+    b.rc2 = hb.cutoff_sqr;
+    b.lcell = hb.lcell;
+    b.celltype = hb.celltype;
+    b.dim = hb.dim;
+    return b;
+}
+
+template <typename T>
+int build_host_box(int dim, const double *cell, int celltype, double cutoff, int lcell, const double *origin,
+                   HostBox<T> &hb) {
+    if (dim != 2 && dim != 3) return BOX_BAD_DIM;
+    T c[9], o[3] = {0, 0, 0};
+    for (int k = 0; k < dim * dim; ++k) c[k] = T(cell[k]);
+    if (celltype == CLM_ORTHORHOMBIC || celltype == CLM_NONPERIODIC)
+        for (int i = 0; i < dim; ++i)
+            for (int j = 0; j < dim; ++j)
+                if (i != j) c[i + dim * j] = T(0);
+    if (origin)
+        for (int k = 0; k < dim; ++k) o[k] = T(origin[k]);
+    return build_box<T>(dim, c, celltype, T(cutoff), lcell, o, hb);
+}
+
+template <typename T>
+void box_info(const HostBox<T> &hb, clm_box_info *out) {
+    std::memset(out, 0, sizeof(*out));
+    out->dim = hb.dim;
+    out->celltype = hb.celltype;
+    out->lcell = hb.lcell;
+    for (int k = 0; k < 3; ++k) {
+        out->nc[k] = hb.nc[k];
+        out->cell_size[k] = hb.cs[k];
+        out->cbmin[k] = hb.cbmin[k];
+        out->cbmax[k] = hb.cbmax[k];
+        out->origin[k] = hb.origin[k];
+    }
+    for (int k = 0; k < 9; ++k) {
+        out->input[k] = hb.input.a[k];
+        out->aligned[k] = hb.aligned.a[k];
+        out->rotation[k] = hb.rot.a[k];
+        out->inv_rotation[k] = hb.invrot.a[k];
+    }
+    out->cutoff = hb.cutoff;
+    out->cutoff_sqr = hb.cutoff_sqr;
+}
+
+clm_status box_status(clm_ctx *c, int rc) {
+    if (rc == BOX_OK) return CLM_OK;
+    if (rc == BOX_BAD_LCELL) return fail(c, CLM_ERR_ARG, "lcell must be greater or equal to 1");
+    if (rc == BOX_BAD_DIM) return fail(c, CLM_ERR_DIM, "dimension must be 2 or 3");
+    return fail(c, CLM_ERR_CELL, "Unit cell matrix does not satisfy required conditions "
+                                 "(distance between cell planes must exceed 2*cutoff)");
+}
+
+// limits(x) + _sides_from_limits (CellOperations.jl:262-297, Box.jl:38) for the
+// non-periodic box; positions are read back to the host for this O(N) reduction.
+template <typename T>
+clm_status nonperiodic_box(clm_ctx *c) {
+    const int D = c->dim;
+    std::vector<T> x((size_t)c->n * D);
+    if (c->n > 0) {
+        HIPCHK(c, hipMemcpyAsync(x.data(), c->pos.p, x.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        clm_status s = sync(c);
+        if (s) return s;
+    }
+    T mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    if (c->n > 0) {
+        for (int k = 0; k < D; ++k) { mn[k] = INFINITY; mx[k] = -INFINITY; }
+        for (int64_t i = 0; i < c->n; ++i)
+            for (int k = 0; k < D; ++k) {
+                T v = x[(size_t)i * D + k];
+                if (v != v) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN)");
+                mn[k] = std::min(mn[k], v);
+                mx[k] = std::max(mx[k], v);
+            }
+    }
+    const T rc = T(c->cutoff_in);
+    const T pad = (T(210) * rc) / T(100);
+    double cell[9] = {0}, origin[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) {
+        cell[k + D * k] = double((mx[k] - mn[k]) + pad);
+        origin[k] = double(mn[k]);
+    }
+    int rcode = build_host_box<T>(D, cell, CLM_NONPERIODIC, c->cutoff_in, c->lcell, origin, hbox<T>(c));
+    return box_status(c, rcode);
+}
+
+template <typename T>
+clm_status update_cell_list_t(clm_ctx *c) {
+    if (!c->has_box) return fail(c, CLM_ERR_STATE, "no box set");
+    if (!c->has_pos) return fail(c, CLM_ERR_STATE, "no positions set");
+    if (c->celltype == CLM_NONPERIODIC) {
+        clm_status s = nonperiodic_box<T>(c);
+        if (s) return s;
+    }
+    HostBox<T> &hb = hbox<T>(c);
+    const int D = c->dim;
+    const int64_t ncells64 = (int64_t)hb.nc[0] * hb.nc[1] * hb.nc[2];
+    if (ncells64 > (int64_t)1 << 30) return fail(c, CLM_ERR_ARG, "too many computing cells");
+    if (c->n >= (int64_t)IDX_MASK) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
+    c->ncells = (int)ncells64;
+    c->nrows = hb.nc[1] * hb.nc[2];
+    const int64_t n = c->n;
+    const int nc1 = c->ncells + 1;
+    HIPCHK(c, ensure(c->cnt, sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->creal, sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->cstart, sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->cur_real, sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->cur_ghost, sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->xw, sizeof(Ent<T>) * std::max<int64_t>(n, 1)));
+    HIPCHK(c, ensure(c->home_cell, sizeof(int) * std::max<int64_t>(n, 1)));
+    HIPCHK(c, ensure(c->flags, sizeof(uint32_t) * 4));
+    clm_status s;
+    if ((s = fill_int(c, c->cnt.p, nc1, 0))) return s;
+    if ((s = fill_int(c, c->creal.p, nc1, 0))) return s;
+    if ((s = fill_int(c, c->flags.p, 4, 0))) return s;
+    const DevBox<T> db = dev_box(hb);
+    const int periodic = c->celltype != CLM_NONPERIODIC;
+    if (n > 0) {
+        if (D == 3)
+            LAUNCH(c, "k_bin_count", hipLaunchKernelGGL((k_bin_count<T, 3>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                                                        (const T *)c->pos.p, (int)n, db, periodic, (Ent<T> *)c->xw.p,
+                                                        (int *)c->home_cell.p, (int *)c->cnt.p, (int *)c->creal.p,
+                                                        (uint32_t *)c->flags.p));
+        else
+            LAUNCH(c, "k_bin_count", hipLaunchKernelGGL((k_bin_count<T, 2>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                                                        (const T *)c->pos.p, (int)n, db, periodic, (Ent<T> *)c->xw.p,
+                                                        (int *)c->home_cell.p, (int *)c->cnt.p, (int *)c->creal.p,
+                                                        (uint32_t *)c->flags.p));
+    }
+    if ((s = scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1))) return s;
+    int total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync(c))) return s;
+    if (*c->h_flags & FLAG_NAN) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN or Inf)");
+    if (*c->h_flags & FLAG_RANGE)
+        return fail(c, CLM_ERR_RANGE, "a particle maps outside the computing grid (the reference raises a BoundsError)");
+    c->n_entries = total;
+    HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * std::max<int64_t>(total, 1)));
+    c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>));
+    if ((s = fill_int(c, c->cur_real.p, nc1, 0))) return s;
+    if ((s = fill_int(c, c->cur_ghost.p, nc1, 0))) return s;
+    if (n > 0) {
+        const int *tp = c->has_types ? (const int *)c->types.p : nullptr;
+        if (D == 3)
+            LAUNCH(c, "k_bin_scatter", hipLaunchKernelGGL((k_bin_scatter<T, 3>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                                                          (int)n, db, periodic, (const Ent<T> *)c->xw.p,
+                                                          (const int *)c->home_cell.p, (const int *)c->cstart.p,
+                                                          (const int *)c->creal.p, tp, (int *)c->cur_real.p,
+                                                          (int *)c->cur_ghost.p, (Ent<T> *)c->E.p, c->e_cap,
+                                                          (uint32_t *)c->flags.p));
+        else
+            LAUNCH(c, "k_bin_scatter", hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                                                          (int)n, db, periodic, (const Ent<T> *)c->xw.p,
+                                                          (const int *)c->home_cell.p, (const int *)c->cstart.p,
+                                                          (const int *)c->creal.p, tp, (int *)c->cur_real.p,
+                                                          (int *)c->cur_ghost.p, (Ent<T> *)c->E.p, c->e_cap,
+                                                          (uint32_t *)c->flags.p));
+    }
+    c->cl_valid = true;
+    c->dirty = false;
+    for (bool &m : c->mode_valid) m = false;
+    return CLM_OK;
+}
+
+clm_status ensure_cell_list(clm_ctx *c) {
+    if (c->cl_valid && !c->dirty) return CLM_OK;
+    return c->dtype == CLM_F32 ? update_cell_list_t<float>(c) : update_cell_list_t<double>(c);
+}
+
+int mode_a(const clm_ctx *c) { return c->celltype == CLM_TRICLINIC ? MODE_A_TRIC : MODE_A_ORTHO; }
+
+clm_status prepare_mode(clm_ctx *c, int mode) {
+    if (c->mode_valid[mode]) return CLM_OK;
+    const int nc1 = c->ncells + 1;
+    HIPCHK(c, ensure(c->hc, sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->hstart[mode], sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->ntask, sizeof(int) * (c->nrows + 1)));
+    HIPCHK(c, ensure(c->tstart[mode], sizeof(int) * (c->nrows + 1)));
+    const int all_entries = (mode == MODE_A_ORTHO) ? 1 : 0;
+    LAUNCH(c, "k_home_counts", hipLaunchKernelGGL(k_home_counts, dim3(blocks(nc1, 256)), dim3(256), 0, c->stream, c->ncells,
+                                                  all_entries, (const int *)c->cstart.p, (const int *)c->creal.p,
+                                                  (int *)c->hc.p));
+    clm_status s;
+    if ((s = scan_int(c, (const int *)c->hc.p, (int *)c->hstart[mode].p, nc1))) return s;
+    const int ncx = (c->dtype == CLM_F32) ? c->hbf.nc[0] : c->hbd.nc[0];
+    LAUNCH(c, "k_row_tasks", hipLaunchKernelGGL(k_row_tasks, dim3(blocks(c->nrows + 1, 256)), dim3(256), 0, c->stream,
+                                                c->nrows, ncx, (const int *)c->hstart[mode].p, (int *)c->ntask.p));
+    if ((s = scan_int(c, (const int *)c->ntask.p, (int *)c->tstart[mode].p, c->nrows + 1))) return s;
+    c->mode_valid[mode] = true;
+    return CLM_OK;
+}
+
+template <typename T>
+SweepArgs<T> sweep_args(clm_ctx *c, int mode) {
+    const HostBox<T> &hb = hbox<T>(c);
+    SweepArgs<T> sa;
+    sa.E = (const Ent<T> *)c->E.p;
+    sa.cstart = (const int *)c->cstart.p;
+    sa.hstart = (const int *)c->hstart[mode].p;
+    sa.tstart = (const int *)c->tstart[mode].p;
+    sa.task_counter = (int *)c->counters.p;
+    for (int k = 0; k < 3; ++k) sa.nc[k] = hb.nc[k];
+    sa.nrows = c->nrows;
+    sa.lcell = hb.lcell;
+    sa.dim = hb.dim;
+    sa.rc2 = hb.cutoff_sqr;
+    return sa;
+}
+
+int sweep_grid(clm_ctx *c) { return std::max(64, c->num_cus * 16); }
+
+template <typename T, int MODE, typename Op>
+clm_status run_sweep(clm_ctx *c, const char *name, const Op &op, int grid) {
+    HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
+    clm_status s;
+    if ((s = fill_int(c, c->counters.p, 16, 0))) return s;
+    SweepArgs<T> sa = sweep_args<T>(c, MODE);
+    if (c->dim == 3)
+        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, sa.E, sa.cstart, sa, op));
+    else
+        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, sa.E, sa.cstart, sa, op));
+    return CLM_OK;
+}
+
+template <typename T, typename Op>
+clm_status run_mode_a(clm_ctx *c, const char *name, const Op &op, int grid) {
+    if (mode_a(c) == MODE_A_TRIC) return run_sweep<T, MODE_A_TRIC>(c, name, op, grid);
+    return run_sweep<T, MODE_A_ORTHO>(c, name, op, grid);
+}
+
+template <typename T>
+clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_t cap, int64_t *npairs, int on_device) {
+    clm_status s;
+    if ((s = ensure_cell_list(c))) return s;
+    const int ma = mode_a(c);
+    if ((s = prepare_mode(c, ma))) return s;
+    int32_t *di = oi, *dj = oj;
+    T *dd = (T *)od;
+    if (!on_device) {
+        HIPCHK(c, ensure(c->nl_i, sizeof(int32_t) * std::max<int64_t>(cap, 1)));
+        HIPCHK(c, ensure(c->nl_j, sizeof(int32_t) * std::max<int64_t>(cap, 1)));
+        HIPCHK(c, ensure(c->nl_d, sizeof(T) * std::max<int64_t>(cap, 1)));
+        di = (int32_t *)c->nl_i.p;
+        dj = (int32_t *)c->nl_j.p;
+        dd = (T *)c->nl_d.p;
+    }
+    HIPCHK(c, ensure(c->nl_count, sizeof(unsigned long long)));
+    if ((s = fill_int(c, c->nl_count.p, 2, 0))) return s;
+    OpNeighborList<T> op;
+    op.oi = di;
+    op.oj = dj;
+    op.od = dd;
+    op.count = (unsigned long long *)c->nl_count.p;
+    op.cap = cap;
+    if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, sweep_grid(c)))) return s;
+    unsigned long long cnt = 0;
+    HIPCHK(c, hipMemcpyAsync(&cnt, c->nl_count.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync(c))) return s;
+    *npairs = (int64_t)cnt;
+    if (!on_device && cnt > 0) {
+        const int64_t m = std::min<int64_t>((int64_t)cnt, cap);
+        HIPCHK(c, hipMemcpyAsync(oi, di, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(oj, dj, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(od, dd, sizeof(T) * m, hipMemcpyDeviceToHost, c->stream));
+        if ((s = sync(c))) return s;
+    }
+    if ((int64_t)cnt > cap) return fail(c, CLM_ERR_CAPACITY, "neighbor list capacity too small");
+    return CLM_OK;
+}
+
+template <typename T>
+SoftParams<T> soft_params(const clm_softcore *p) {
+    SoftParams<T> sp;
+    std::memset(&sp, 0, sizeof(sp));
+    const int nt = p->ntypes > 0 ? p->ntypes : 1;
+    sp.ntypes = p->ntypes > 0 ? p->ntypes : 0;
+    for (int a = 0; a < nt; ++a) {
+        sp.eps_t[a] = T(p->eps[a]);
+        for (int b = 0; b < nt; ++b) {
+            const int t = sp.ntypes ? a * 8 + b : 0;
+            const double e = std::sqrt(p->eps[a] * p->eps[b]);
+            const double sg = (p->sigma[a] + p->sigma[b]) / 2.0;
+            sp.eps[t] = T(e);
+            sp.sig[t] = T(sg);
+            sp.sig2[t] = T(sg * sg);
+        }
+    }
+    return sp;
+}
+
+template <typename T, bool FORCES, bool ADJ>
+clm_status softcore_t(clm_ctx *c, const clm_softcore *p, double *energy, void *fout, double *deps, double *dsig,
+                      int on_device, int64_t *npairs) {
+    clm_status s;
+    if (!p) return fail(c, CLM_ERR_ARG, "NULL soft-core parameters");
+    if (p->ntypes < 0 || p->ntypes > 8) return fail(c, CLM_ERR_ARG, "ntypes must be in 0..8");
+    if (p->ntypes > 0 && !c->has_types) return fail(c, CLM_ERR_STATE, "typed parameters need clm_set_types");
+    if ((s = ensure_cell_list(c))) return s;
+    if ((s = prepare_mode(c, MODE_B))) return s;
+    const int D = c->dim;
+    T *df = (T *)fout;
+    if ((FORCES || ADJ) && (!on_device || !fout)) {
+        HIPCHK(c, ensure(c->outbuf, sizeof(T) * std::max<int64_t>(c->n * D, 1)));
+        df = (T *)c->outbuf.p;
+    }
+    HIPCHK(c, ensure(c->acc, sizeof(double) * 32));
+    if ((s = fill_int(c, c->acc.p, 64, 0))) return s;
+    OpSoftCore<T, FORCES, ADJ> op;
+    op.prm = soft_params<T>(p);
+    op.forces = df;
+    op.energy = (double *)c->acc.p;
+    op.deps = (double *)c->acc.p + 8;
+    op.dsig = (double *)c->acc.p + 16;
+    const HostBox<T> &hb = hbox<T>(c);
+    for (int k = 0; k < 9; ++k) op.invrot[k] = hb.invrot.a[k];
+    op.tric = c->celltype == CLM_TRICLINIC;
+    op.dim = D;
+    const char *name = ADJ ? "k_sweep_energy_adjoint" : (FORCES ? "k_sweep_energy_forces" : "k_sweep_energy");
+    if ((s = run_sweep<T, MODE_B>(c, name, op, sweep_grid(c)))) return s;
+    double acc[24];
+    HIPCHK(c, hipMemcpyAsync(acc, c->acc.p, sizeof(double) * 24, hipMemcpyDeviceToHost, c->stream));
+    if ((FORCES || ADJ) && fout && !on_device && c->n > 0)
+        HIPCHK(c, hipMemcpyAsync(fout, df, sizeof(T) * c->n * D, hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync(c))) return s;
+    if (energy) *energy = acc[0];
+    const int nt = p->ntypes > 0 ? p->ntypes : 1;
+    if (ADJ) {
+        for (int k = 0; k < nt; ++k) {
+            if (deps) deps[k] = acc[8 + k];
+            if (dsig) dsig[k] = acc[16 + k];
+        }
+    }
+    if (npairs) *npairs = -1;
+    return CLM_OK;
+}
+
+template <typename T>
+clm_status histogram_t(clm_ctx *c, int nbins, double width, int64_t *counts, double *vsum) {
+    clm_status s;
+    if (nbins < 1 || nbins > OpHistogram<T>::MAXB) return fail(c, CLM_ERR_ARG, "nbins must be in 1..512");
+    if (!(width > 0)) return fail(c, CLM_ERR_ARG, "bin width must be positive");
+    if (vsum && !c->has_vel) return fail(c, CLM_ERR_STATE, "velocities not set");
+    if ((s = ensure_cell_list(c))) return s;
+    const int ma = mode_a(c);
+    if ((s = prepare_mode(c, ma))) return s;
+    HIPCHK(c, ensure(c->hist_c, sizeof(unsigned long long) * nbins));
+    HIPCHK(c, ensure(c->hist_v, sizeof(double) * nbins));
+    if ((s = fill_int(c, c->hist_c.p, 2 * nbins, 0))) return s;
+    if ((s = fill_int(c, c->hist_v.p, 2 * nbins, 0))) return s;
+    OpHistogram<T> op;
+    op.nbins = nbins;
+    op.width = T(width);
+    op.vel = vsum ? (const T *)c->vel.p : nullptr;
+    const HostBox<T> &hb = hbox<T>(c);
+    for (int k = 0; k < 9; ++k) op.rot[k] = hb.rot.a[k];
+    op.tric = c->celltype == CLM_TRICLINIC;
+    op.dim = c->dim;
+    op.counts = (unsigned long long *)c->hist_c.p;
+    op.vsum = (double *)c->hist_v.p;
+    if ((s = run_mode_a<T>(c, "k_sweep_histogram", op, sweep_grid(c)))) return s;
+    std::vector<unsigned long long> hc(nbins);
+    HIPCHK(c, hipMemcpyAsync(hc.data(), c->hist_c.p, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost, c->stream));
+    if (vsum) HIPCHK(c, hipMemcpyAsync(vsum, c->hist_v.p, sizeof(double) * nbins, hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync(c))) return s;
+    for (int k = 0; k < nbins; ++k) counts[k] = (int64_t)hc[k];
+    return CLM_OK;
+}
+
+template <typename T>
+clm_status mindist_t(clm_ctx *c, int64_t *oi, int64_t *oj, double *od) {
+    clm_status s;
+    if ((s = ensure_cell_list(c))) return s;
+    const int ma = mode_a(c);
+    if ((s = prepare_mode(c, ma))) return s;
+    const int grid = sweep_grid(c);
+    HIPCHK(c, ensure(c->md_d2, sizeof(T) * grid));
+    HIPCHK(c, ensure(c->md_a, sizeof(int) * grid));
+    HIPCHK(c, ensure(c->md_b, sizeof(int) * grid));
+    OpMinDist<T> op;
+    op.wd2 = (T *)c->md_d2.p;
+    op.wa = (int *)c->md_a.p;
+    op.wb = (int *)c->md_b.p;
+    if ((s = run_mode_a<T>(c, "k_sweep_mindist", op, grid))) return s;
+    std::vector<T> d2(grid);
+    std::vector<int> a(grid), b(grid);
+    HIPCHK(c, hipMemcpyAsync(d2.data(), c->md_d2.p, sizeof(T) * grid, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(a.data(), c->md_a.p, sizeof(int) * grid, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.data(), c->md_b.p, sizeof(int) * grid, hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync(c))) return s;
+    T best = T(INFINITY);
+    int ba = 0x7fffffff, bb = 0x7fffffff;
+    for (int k = 0; k < grid; ++k)
+        if (OpMinDist<T>::better(d2[k], a[k], b[k], best, ba, bb)) { best = d2[k]; ba = a[k]; bb = b[k]; }
+    if (std::isinf(best)) {
+        *oi = -1; *oj = -1; *od = INFINITY;
+    } else {
+        *oi = ba; *oj = bb; *od = (double)std::sqrt(best);
+    }
+    return CLM_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------
+
+extern "C" {
+
+const char *clm_version(void) { return "celllistmap-hip 0.1.0 (gfx950)"; }
+
+clm_status clm_create(clm_ctx **out, int device, int dtype, int dim) {
+    if (!out) return CLM_ERR_ARG;
+    *out = nullptr;
+    if (dtype != CLM_F32 && dtype != CLM_F64) return CLM_ERR_ARG;
+    if (dim != 2 && dim != 3) return CLM_ERR_DIM;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return CLM_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return CLM_ERR_HIP;
+    clm_ctx *c = new clm_ctx();
+    c->device = device;
+    c->dtype = dtype;
+    c->dim = dim;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CLM_ERR_HIP; }
+    c->own_stream = true;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) c->num_cus = prop.multiProcessorCount;
+    if (hipHostMalloc((void **)&c->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return CLM_ERR_HIP;
+    }
+    *out = c;
+    return CLM_OK;
+}
+
+clm_status clm_destroy(clm_ctx *c) {
+    if (!c) return CLM_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    DBuf *bufs[] = {&c->pos, &c->types, &c->vel, &c->xw, &c->home_cell, &c->cnt, &c->creal, &c->cstart,
+                    &c->cur_real, &c->cur_ghost, &c->E, &c->flags, &c->hc, &c->ntask, &c->hstart[0],
+                    &c->hstart[1], &c->hstart[2], &c->tstart[0], &c->tstart[1], &c->tstart[2], &c->counters,
+                    &c->scan_tmp, &c->nl_i, &c->nl_j, &c->nl_d, &c->nl_count, &c->outbuf, &c->acc,
+                    &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b};
+    for (DBuf *b : bufs)
+        if (b->p) hipFree(b->p);
+    for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->h_flags) hipHostFree(c->h_flags);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return CLM_OK;
+}
+
+const char *clm_last_error(const clm_ctx *c) { return c ? c->err.c_str() : "NULL context"; }
+
+clm_status clm_set_stream(clm_ctx *c, void *stream) {
+    if (!c) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    if (stream) {
+        if (c->own_stream && c->stream) {
+            hipStreamSynchronize(c->stream);
+            hipStreamDestroy(c->stream);
+        }
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else if (!c->own_stream) {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return CLM_ERR_HIP;
+        c->own_stream = true;
+    }
+    return CLM_OK;
+}
+
+clm_status clm_box_compute(int dtype, int dim, const double *cell, int celltype, double cutoff, int lcell,
+                           const double *origin, clm_box_info *out) {
+    if (!cell || !out) return CLM_ERR_ARG;
+    if (celltype < 0 || celltype > 2) return CLM_ERR_ARG;
+    int rc;
+    if (dtype == CLM_F32) {
+        HostBox<float> hb;
+        rc = build_host_box<float>(dim, cell, celltype, cutoff, lcell, origin, hb);
+        if (rc == BOX_OK) box_info(hb, out);
+    } else if (dtype == CLM_F64) {
+        HostBox<double> hb;
+        rc = build_host_box<double>(dim, cell, celltype, cutoff, lcell, origin, hb);
+        if (rc == BOX_OK) box_info(hb, out);
+    } else {
+        return CLM_ERR_ARG;
+    }
+    if (rc == BOX_BAD_LCELL) return CLM_ERR_ARG;
+    if (rc == BOX_BAD_DIM) return CLM_ERR_DIM;
+    if (rc == BOX_BAD_CELL) return CLM_ERR_CELL;
+    return CLM_OK;
+}
+
+clm_status clm_set_box(clm_ctx *c, const double *cell, int celltype, double cutoff, int lcell) {
+    if (!c) return CLM_ERR_ARG;
+    if (celltype < 0 || celltype > 2) return fail(c, CLM_ERR_ARG, "unknown cell type");
+    if (lcell < 1) return fail(c, CLM_ERR_ARG, "lcell must be greater or equal to 1");
+    if (!(cutoff > 0) || !std::isfinite(cutoff)) return fail(c, CLM_ERR_ARG, "cutoff must be positive and finite");
+    const int D = c->dim;
+    if (celltype != CLM_NONPERIODIC) {
+        if (!cell) return fail(c, CLM_ERR_ARG, "NULL unit cell");
+        int rc = c->dtype == CLM_F32 ? build_host_box<float>(D, cell, celltype, cutoff, lcell, nullptr, c->hbf)
+                                     : build_host_box<double>(D, cell, celltype, cutoff, lcell, nullptr, c->hbd);
+        clm_status s = box_status(c, rc);
+        if (s) return s;
+        for (int k = 0; k < D * D; ++k) c->cell_in[k] = cell[k];
+    }
+    c->celltype = celltype;
+    c->cutoff_in = cutoff;
+    c->lcell = lcell;
+    c->has_box = true;
+    c->dirty = true;
+    return CLM_OK;
+}
+
+clm_status clm_get_box(clm_ctx *c, clm_box_info *out) {
+    if (!c || !out) return CLM_ERR_ARG;
+    if (!c->has_box) return fail(c, CLM_ERR_STATE, "no box set");
+    if (c->celltype == CLM_NONPERIODIC) {
+        clm_status s = ensure_cell_list(c);
+        if (s) return s;
+    }
+    if (c->dtype == CLM_F32) box_info(c->hbf, out);
+    else box_info(c->hbd, out);
+    return CLM_OK;
+}
+
+clm_status clm_set_positions(clm_ctx *c, const void *x, int64_t n, int on_device) {
+    if (!c) return CLM_ERR_ARG;
+    if (n < 0 || (n > 0 && !x)) return fail(c, CLM_ERR_ARG, "bad positions");
+    hipSetDevice(c->device);
+    const size_t es = c->dtype == CLM_F32 ? 4 : 8;
+    const size_t bytes = es * (size_t)c->dim * (size_t)n;
+    HIPCHK(c, ensure(c->pos, std::max<size_t>(bytes, 16)));
+    if (bytes)
+        HIPCHK(c, hipMemcpyAsync(c->pos.p, x, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (!on_device) {
+        clm_status s = sync(c);
+        if (s) return s;
+    }
+    if (n != c->n) c->has_types = false;
+    c->n = n;
+    c->has_pos = true;
+    c->dirty = true;
+    return CLM_OK;
+}
+
+clm_status clm_set_types(clm_ctx *c, const int32_t *t, int64_t n, int on_device) {
+    if (!c) return CLM_ERR_ARG;
+    if (!c->has_pos || n != c->n) return fail(c, CLM_ERR_DIM, "types must have one entry per particle");
+    HIPCHK(c, ensure(c->types, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    if (n) HIPCHK(c, hipMemcpyAsync(c->types.p, t, sizeof(int32_t) * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (!on_device) {
+        clm_status s = sync(c);
+        if (s) return s;
+    }
+    c->has_types = true;
+    c->dirty = true;
+    return CLM_OK;
+}
+
+clm_status clm_set_velocities(clm_ctx *c, const void *v, int64_t n, int on_device) {
+    if (!c) return CLM_ERR_ARG;
+    if (!c->has_pos || n != c->n) return fail(c, CLM_ERR_DIM, "velocities must have one entry per particle");
+    const size_t es = c->dtype == CLM_F32 ? 4 : 8;
+    const size_t bytes = es * c->dim * (size_t)n;
+    HIPCHK(c, ensure(c->vel, std::max<size_t>(bytes, 16)));
+    if (bytes) HIPCHK(c, hipMemcpyAsync(c->vel.p, v, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (!on_device) {
+        clm_status s = sync(c);
+        if (s) return s;
+    }
+    c->has_vel = true;
+    return CLM_OK;
+}
+
+clm_status clm_update_cell_list(clm_ctx *c) {
+    if (!c) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    c->dirty = true;
+    clm_status s = ensure_cell_list(c);
+    if (s) return s;
+    return sync(c);
+}
+
+clm_status clm_cell_list_info(clm_ctx *c, int64_t *out4) {
+    if (!c || !out4) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    clm_status s = ensure_cell_list(c);
+    if (s) return s;
+    std::vector<int> cr(c->ncells), cst(c->ncells + 1);
+    HIPCHK(c, hipMemcpyAsync(cr.data(), c->creal.p, sizeof(int) * c->ncells, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cst.data(), c->cstart.p, sizeof(int) * (c->ncells + 1), hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync(c))) return s;
+    int64_t withreal = 0, withp = 0;
+    for (int k = 0; k < c->ncells; ++k) {
+        withreal += cr[k] > 0;
+        withp += cst[k + 1] > cst[k];
+    }
+    out4[0] = cst[c->ncells];
+    out4[1] = withreal;
+    out4[2] = withp;
+    out4[3] = c->ncells;
+    return CLM_OK;
+}
+
+clm_status clm_neighborlist(clm_ctx *c, int32_t *i, int32_t *j, void *d, int64_t capacity, int64_t *npairs, int on_device) {
+    if (!c || !npairs || capacity < 0) return CLM_ERR_ARG;
+    if (capacity > 0 && (!i || !j || !d)) return fail(c, CLM_ERR_ARG, "NULL output arrays");
+    hipSetDevice(c->device);
+    return c->dtype == CLM_F32 ? neighborlist_t<float>(c, i, j, d, capacity, npairs, on_device)
+                               : neighborlist_t<double>(c, i, j, d, capacity, npairs, on_device);
+}
+
+clm_status clm_energy_forces(clm_ctx *c, const clm_softcore *p, double *energy, void *forces, int on_device, int64_t *npairs) {
+    if (!c) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    if (forces)
+        return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energy, forces, nullptr, nullptr, on_device, npairs)
+                                   : softcore_t<double, true, false>(c, p, energy, forces, nullptr, nullptr, on_device, npairs);
+    return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energy, nullptr, nullptr, nullptr, on_device, npairs)
+                               : softcore_t<double, false, false>(c, p, energy, nullptr, nullptr, nullptr, on_device, npairs);
+}
+
+clm_status clm_energy_adjoint(clm_ctx *c, const clm_softcore *p, double *energy, void *dedx, double *deps, double *dsig,
+                              int on_device) {
+    if (!c) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    return c->dtype == CLM_F32 ? softcore_t<float, true, true>(c, p, energy, dedx, deps, dsig, on_device, nullptr)
+                               : softcore_t<double, true, true>(c, p, energy, dedx, deps, dsig, on_device, nullptr);
+}
+
+clm_status clm_histogram(clm_ctx *c, int nbins, double width, int64_t *counts, double *vsum) {
+    if (!c || !counts) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    return c->dtype == CLM_F32 ? histogram_t<float>(c, nbins, width, counts, vsum)
+                               : histogram_t<double>(c, nbins, width, counts, vsum);
+}
+
+clm_status clm_min_distance(clm_ctx *c, int64_t *i, int64_t *j, double *d) {
+    if (!c || !i || !j || !d) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    return c->dtype == CLM_F32 ? mindist_t<float>(c, i, j, d) : mindist_t<double>(c, i, j, d);
+}
+
+clm_status clm_profile_enable(clm_ctx *c, int enable) {
+    if (!c) return CLM_ERR_ARG;
+    c->prof = enable != 0;
+    return CLM_OK;
+}
+
+clm_status clm_profile_get(clm_ctx *c, const char *name, double *total_ms, int64_t *count) {
+    if (!c || !name || !total_ms || !count) return CLM_ERR_ARG;
+    if (c->prof && !c->pending.empty()) {
+        hipStreamSynchronize(c->stream);
+        collect_profile(c);
+    }
+    auto it = c->prof_acc.find(name);
+    *total_ms = it == c->prof_acc.end() ? 0.0 : it->second.first;
+    *count = it == c->prof_acc.end() ? 0 : it->second.second;
+    return CLM_OK;
+}
+
+clm_status clm_profile_reset(clm_ctx *c) {
+    if (!c) return CLM_ERR_ARG;
+    if (!c->pending.empty()) {
+        hipStreamSynchronize(c->stream);
+        collect_profile(c);
+    }
+    c->prof_acc.clear();
+    return CLM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,540 @@
+// clm_kernels.hpp -- HIP kernels of the cell-list engine for gfx950.
+//
+//   k_bin_count   wrap + rotate each particle, count it and its ghost images per cell
+//                 (add_particles! / replicate_particle!, CellLists.jl:940-1057, Box.jl:556-566)
+//   k_bin_scatter write the cell-sorted entries (reals first in each cell)
+//   k_home_counts / k_row_tasks   per-cell home counts and per-row wave tasks
+//   k_sweep<Op>   the cell-pair sweep (self.jl:106-184, vicinal_cells.jl:4-75) with a
+//                 fixed device operator Op in place of the user closure
+//
+// Sweep mapping: one wave (64 lanes) takes up to 64 consecutive home entries of one
+// x-row of cells.  For each of the (2L+1)^2 (3D) stencil rows the union of the lanes'
+// x-ranges is one contiguous slice of E; every lane walks the same entry j (a
+// wave-uniform load) and keeps it if j lies in its own stencil range, the pair filter
+// of the mode holds and d2 <= rc^2.  Modes:
+//   MODE_A_ORTHO  each pair once, reference orientation for orthorhombic/non-periodic
+//                 boxes: homes = all entries of cells holding a real particle, forward
+//                 stencil, later entries of the own cell, skip ghost-ghost pairs.
+//   MODE_A_TRIC   each pair once for triclinic boxes: homes = real particles, full
+//                 stencil, keep index_i < index_j.
+//   MODE_B        each pair from both sides (homes = real particles, full stencil,
+//                 index_i != index_j); operators accumulate only into the home
+//                 particle, so no atomics are needed for forces.
+#pragma once
+
+#include "clm_device.hpp"
+
+namespace clm {
+
+enum { MODE_A_ORTHO = 0, MODE_A_TRIC = 1, MODE_B = 2 };
+enum { NONPER_FLAG = 2 };
+
+// ------------------------------------------------------------------------------
+// binning
+// ------------------------------------------------------------------------------
+
+template <typename T, int D>
+__global__ void __launch_bounds__(256) k_bin_count(const T *__restrict__ x, int n, DevBox<T> b, int periodic,
+                                                   Ent<T> *__restrict__ xw, int *__restrict__ home_cell,
+                                                   int *__restrict__ cnt, int *__restrict__ creal,
+                                                   uint32_t *__restrict__ gflags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T xi[3] = {T(0), T(0), T(0)};
+    bool finite = true;
+    for (int k = 0; k < D; ++k) {
+        xi[k] = x[(size_t)i * D + k];
+        finite = finite && dev_finite(xi[k]);
+    }
+    uint32_t flags = 0;
+    if (!finite) {
+        atomicOr(gflags, FLAG_NAN);
+        home_cell[i] = -1;
+        return;
+    }
+    T p[3] = {T(0), T(0), T(0)};
+    if (periodic) {
+        if (D == 3) wrap_rotate3(b, xi, p);
+        else wrap_rotate2(b, xi, p);
+    } else {
+        for (int k = 0; k < D; ++k) p[k] = xi[k];
+    }
+    Ent<T> e;
+    e.x = p[0]; e.y = p[1]; e.z = p[2];
+    e.w = (uint32_t)i;
+    xw[i] = e;
+    int c = cell_of(b, p, true, flags);
+    home_cell[i] = c;
+    atomicAdd(&cnt[c], 1);
+    atomicAdd(&creal[c], 1);
+    if (periodic) {
+        const int nimg = D == 3 ? 27 : 9;
+        for (int m = 0; m < nimg; ++m) {
+            if (m == (D == 3 ? 13 : 4)) continue;
+            T q[3] = {p[0], p[1], p[2]};
+            for (int k = 0; k < D; ++k) q[k] = p[k] + b.shift[m][k];
+            if (!in_box(b, q)) continue;
+            int cj = cell_of(b, q, false, flags);
+            atomicAdd(&cnt[cj], 1);
+        }
+    }
+    if (flags) atomicOr(gflags, flags);
+}
+
+template <typename T, int D>
+__global__ void __launch_bounds__(256) k_bin_scatter(int n, DevBox<T> b, int periodic, const Ent<T> *__restrict__ xw,
+                                                     const int *__restrict__ home_cell, const int *__restrict__ cstart,
+                                                     const int *__restrict__ creal, const int *__restrict__ types,
+                                                     int *__restrict__ cur_real, int *__restrict__ cur_ghost,
+                                                     Ent<T> *__restrict__ E, int64_t ecap, uint32_t *__restrict__ gflags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = home_cell[i];
+    if (c < 0) return;
+    uint32_t flags = 0;
+    Ent<T> e = xw[i];
+    uint32_t tbits = types ? ((uint32_t)types[i] & 7u) << TYPE_SHIFT : 0u;
+    e.w = (uint32_t)i | tbits;
+    int slot = atomicAdd(&cur_real[c], 1);
+    int64_t pos = (int64_t)cstart[c] + slot;
+    if (pos < ecap) E[pos] = e;
+    else flags |= FLAG_OVERFLOW;
+    if (periodic) {
+        T p[3] = {e.x, e.y, e.z};
+        const int nimg = D == 3 ? 27 : 9;
+        for (int m = 0; m < nimg; ++m) {
+            if (m == (D == 3 ? 13 : 4)) continue;
+            T q[3] = {p[0], p[1], p[2]};
+            for (int k = 0; k < D; ++k) q[k] = p[k] + b.shift[m][k];
+            if (!in_box(b, q)) continue;
+            uint32_t fl2 = 0;
+            int cj = cell_of(b, q, false, fl2);
+            int s2 = atomicAdd(&cur_ghost[cj], 1);
+            int64_t pj = (int64_t)cstart[cj] + creal[cj] + s2;
+            Ent<T> g;
+            g.x = q[0]; g.y = q[1]; g.z = q[2];
+            g.w = (uint32_t)i | tbits | GHOST_BIT;
+            if (pj < ecap) E[pj] = g;
+            else flags |= FLAG_OVERFLOW;
+        }
+    }
+    if (flags) atomicOr(gflags, flags);
+}
+
+// hc[c] = homes of cell c for the sweep mode; the trailing element is 0 for the scan.
+__global__ void k_home_counts(int ncells, int all_entries, const int *__restrict__ cstart,
+                              const int *__restrict__ creal, int *__restrict__ hc) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncells) return;
+    if (c == ncells) { hc[c] = 0; return; }
+    int nr = creal[c];
+    hc[c] = all_entries ? (nr > 0 ? cstart[c + 1] - cstart[c] : 0) : nr;
+}
+
+__global__ void k_row_tasks(int nrows, int ncx, const int *__restrict__ hstart, int *__restrict__ ntask) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nrows) return;
+    if (r == nrows) { ntask[r] = 0; return; }
+    int h = hstart[(r + 1) * ncx] - hstart[r * ncx];
+    ntask[r] = (h + 63) >> 6;
+}
+
+__global__ void k_fill_int(int *__restrict__ p, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------
+// sweep skeleton
+// ------------------------------------------------------------------------------
+
+template <typename T>
+struct SweepArgs {
+    const Ent<T> *E;
+    const int *cstart;
+    const int *hstart;
+    const int *tstart;
+    int *task_counter;   // [8] per-XCD counters + [8] done flags, zeroed before launch
+    int nc[3];
+    int nrows;
+    int lcell;
+    int dim;
+    T rc2;
+};
+
+// Largest k in [lo, hi) with a[k] <= v (a non-decreasing, a[lo] <= v).
+__device__ inline int last_le(const int *__restrict__ a, int lo, int hi, int v) {
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Dynamic task fetch: blocks are dealt to XCDs round-robin (block b runs on XCD b % 8),
+// so each XCD walks its own contiguous share of the tasks first (neighbouring rows share
+// entries in that XCD's L2) and then helps the others.
+__device__ inline int fetch_task(int *counters, int ntasks, int &home_xcd, int &probe) {
+    while (probe < 8) {
+        int x = (home_xcd + probe) & 7;
+        int lo = (int)(((int64_t)ntasks * x) >> 3);
+        int hi = (int)(((int64_t)ntasks * (x + 1)) >> 3);
+        int t = lo + atomicAdd(&counters[x], 1);
+        if (t < hi) return t;
+        ++probe;
+    }
+    return -1;
+}
+
+template <typename T, int MODE, int D, typename Op>
+__global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, const int *__restrict__ cs,
+                                              const SweepArgs<T> sa, Op op) {
+    __shared__ typename Op::Shared sh;
+    __shared__ int s_task;
+    const int lane = threadIdx.x;
+    op.wave_init(sh);
+    const int ncx = sa.nc[0], ncy = sa.nc[1];
+    const int L = sa.lcell;
+    const int ntasks = sa.tstart[sa.nrows];
+    int home_xcd = blockIdx.x & 7, probe = 0;
+    while (true) {
+        if (lane == 0) s_task = fetch_task(sa.task_counter, ntasks, home_xcd, probe);
+        __syncthreads();
+        const int task = uniform_int(s_task);
+        __syncthreads();
+        if (task < 0) break;
+        const int r = last_le(sa.tstart, 0, sa.nrows + 1, task);
+        const int chunk = task - sa.tstart[r];
+        const int rowc0 = r * ncx;
+        const int hb = sa.hstart[rowc0] + chunk * 64;
+        const int he = sa.hstart[rowc0 + ncx];
+        const int h = hb + lane;
+        const bool valid = h < he;
+        const int c = last_le(sa.hstart, rowc0, rowc0 + ncx, valid ? h : hb);
+        const int pos = cs[c] + ((valid ? h : hb) - sa.hstart[c]);
+        const Ent<T> p = E[pos];
+        const int cx = c - rowc0;
+        const int cy = r % ncy, cz = r / ncy;
+        const int nvalid = min(64, he - hb);
+        const int cxmin = uniform_int(cx);
+        const int cxmax = __shfl(cx, nvalid - 1, 64);
+        const uint32_t wp = ent_w(p);
+        const uint32_t ip = wp & IDX_MASK;
+        const bool preal = !(wp & GHOST_BIT);
+        typename Op::Lane ls;
+        op.lane_init(ls, p, valid);
+        const int dzr = (D == 3) ? L : 0;
+        for (int dz = -dzr; dz <= dzr; ++dz) {
+            for (int dy = -L; dy <= L; ++dy) {
+                const int base = ((cz + dz) * ncy + (cy + dy)) * ncx;
+                int xlo = -L;
+                const bool selfrow = (dz == 0 && dy == 0);
+                if (MODE == MODE_A_ORTHO) xlo = (dy > 0 || (dy == 0 && dz > 0)) ? 0 : 1;
+                int lo = cs[base + cx + xlo];
+                int hi = cs[base + cx + L + 1];
+                if (MODE == MODE_A_ORTHO && selfrow) lo = pos + 1;
+                if (!valid) { lo = 0; hi = 0; }
+                int J0 = uniform_int(cs[base + cxmin + (MODE == MODE_A_ORTHO && selfrow ? 0 : xlo)]);
+                int J1 = uniform_int(__shfl(cs[base + cx + L + 1], nvalid - 1, 64));
+                (void)cxmax;
+                for (int j = J0; j < J1; ++j) {
+                    const Ent<T> q = E[j];
+                    const uint32_t wq = ent_w(q);
+                    const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
+                    T d2 = dx * dx + dy2 * dy2;
+                    if (D == 3) d2 = d2 + dz2 * dz2;
+                    bool ok = d2 <= sa.rc2;
+                    if (MODE == MODE_A_ORTHO) {
+                        ok = ok && (j >= lo) && (j < hi) && (preal || !(wq & GHOST_BIT));
+                    } else if (MODE == MODE_A_TRIC) {
+                        ok = ok && (j >= lo) && (j < hi) && (ip < (wq & IDX_MASK));
+                    } else {
+                        ok = ok && valid && (ip != (wq & IDX_MASK));
+                    }
+                    op.candidate(sh, ls, ok, p, q, d2);
+                }
+            }
+        }
+        op.lane_end(sh, ls, p, valid);
+    }
+    op.wave_end(sh);
+}
+
+// ------------------------------------------------------------------------------
+// operators
+// ------------------------------------------------------------------------------
+
+// Neighbour list: hits are staged in LDS and flushed in coalesced blocks; one global
+// atomic per flush reserves the output range (push_pair!, internals/neighborlist.jl:67-76).
+template <typename T>
+struct OpNeighborList {
+    static constexpr int S = 512;
+    struct Shared {
+        int i[S];
+        int j[S];
+        T d2[S];
+        unsigned long long base;
+    };
+    struct Lane {
+        int n;  // staged hits (same value in every lane)
+    };
+    int32_t *oi;
+    int32_t *oj;
+    T *od;
+    unsigned long long *count;
+    int64_t cap;
+    int staged;  // staged hits carried from one task to the next (kept in a register)
+
+    __device__ void wave_init(Shared &) { staged = 0; }
+    __device__ void lane_init(Lane &l, const Ent<T> &, bool) { l.n = staged; }
+    __device__ void flush(Shared &sh, int n) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh.base = atomicAdd(count, (unsigned long long)n);
+        __syncthreads();
+        const long long base = (long long)sh.base;
+        for (int k = threadIdx.x; k < n; k += 64) {
+            long long q = base + k;
+            if (q < cap) {
+                oi[q] = sh.i[k];
+                oj[q] = sh.j[k];
+                od[q] = dev_sqrt_rn(sh.d2[k]);
+            }
+        }
+        __syncthreads();
+    }
+    __device__ void candidate(Shared &sh, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+        const uint64_t m = ballot64(ok);
+        if (m == 0) return;
+        if (ok) {
+            const int k = l.n + rank_in(m);
+            sh.i[k] = (int)(ent_w(p) & IDX_MASK);
+            sh.j[k] = (int)(ent_w(q) & IDX_MASK);
+            sh.d2[k] = d2;
+        }
+        l.n += __popcll(m);
+        if (l.n > S - 64) {
+            flush(sh, l.n);
+            l.n = 0;
+        }
+    }
+    __device__ void lane_end(Shared &, Lane &l, const Ent<T> &, bool) { staged = l.n; }
+    __device__ void wave_end(Shared &sh) {
+        if (staged > 0) flush(sh, staged);
+    }
+};
+
+// Soft-core parameters for the device: pair tables of eps_ij and sigma_ij^2 (LB mixing).
+template <typename T>
+struct SoftParams {
+    T eps[64];
+    T sig2[64];
+    T sig[64];
+    T eps_t[8];
+    int ntypes;  // 0 = untyped (table entry 0)
+};
+
+// Energy (+ forces) from both sides: each lane owns one real particle, u/2 per visit.
+template <typename T, bool FORCES, bool ADJ>
+struct OpSoftCore {
+    struct Shared {
+        double gde[8];
+        double gds[8];
+    };
+    struct Lane {
+        T fx, fy, fz;
+        T e;
+        T aeps, asig;
+        int ti;
+    };
+    SoftParams<T> prm;
+    T *forces;            // n*D, input frame
+    double *energy;       // global accumulator
+    double *deps, *dsig;  // per-type adjoint accumulators (ADJ)
+    T invrot[9];
+    int tric;
+    int dim;
+
+    __device__ void wave_init(Shared &sh) {
+        if (threadIdx.x < 8) { sh.gde[threadIdx.x] = 0; sh.gds[threadIdx.x] = 0; }
+        __syncthreads();
+    }
+    __device__ void lane_init(Lane &l, const Ent<T> &p, bool) {
+        l.fx = l.fy = l.fz = T(0);
+        l.e = T(0);
+        l.aeps = l.asig = T(0);
+        l.ti = prm.ntypes ? (int)((ent_w(p) >> TYPE_SHIFT) & 7u) : 0;
+    }
+    __device__ void candidate(Shared &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+        if (!ok) return;
+        int t = 0;
+        if (prm.ntypes) t = l.ti * 8 + (int)((ent_w(q) >> TYPE_SHIFT) & 7u);
+        const T e = prm.eps[t];
+        const T inv = T(1) / (d2 + T(1));
+        const T s = prm.sig2[t] * inv;
+        const T s3 = s * s * s;
+        const T s6 = s3 * s3;
+        l.e += e * (s6 - s3);
+        const T w = T(6) * s6 - T(3) * s3;
+        if (FORCES) {
+            const T g = T(2) * e * w * inv;
+            l.fx = fma(g, p.x - q.x, l.fx);
+            l.fy = fma(g, p.y - q.y, l.fy);
+            l.fz = fma(g, p.z - q.z, l.fz);
+        }
+        if (ADJ) {
+            l.aeps += (s6 - s3) * e;
+            l.asig += e * w / prm.sig[t];
+        }
+    }
+    __device__ void lane_end(Shared &sh, Lane &l, const Ent<T> &p, bool valid) {
+        double ev = valid ? 0.5 * (double)l.e : 0.0;
+        ev = wave_sum(ev);
+        if (threadIdx.x == 0) atomicAdd(energy, ev);
+        if (valid) {
+            const uint32_t i = ent_w(p) & IDX_MASK;
+            if (FORCES) {
+                T f[3] = {l.fx, l.fy, l.fz};
+                if (tric) {
+                    T g[3];
+                    for (int r = 0; r < 3; ++r) g[r] = (invrot[r] * f[0] + invrot[r + 3] * f[1]) + invrot[r + 6] * f[2];
+                    for (int r = 0; r < 3; ++r) f[r] = g[r];
+                }
+                for (int k = 0; k < dim; ++k) forces[(size_t)i * dim + k] = ADJ ? -f[k] : f[k];
+            }
+            if (ADJ) {
+                atomicAdd(&sh.gde[l.ti], (double)l.aeps / (2.0 * (double)prm.eps_t[l.ti]));
+                atomicAdd(&sh.gds[l.ti], (double)l.asig);
+            }
+        }
+    }
+    __device__ void wave_end(Shared &sh) {
+        __syncthreads();
+        if (ADJ && threadIdx.x < 8 && (threadIdx.x < (prm.ntypes ? prm.ntypes : 1))) {
+            atomicAdd(&deps[threadIdx.x], sh.gde[threadIdx.x]);
+            atomicAdd(&dsig[threadIdx.x], sh.gds[threadIdx.x]);
+        }
+    }
+};
+
+// Distance histogram (+ radial velocity sums), LDS-privatised per workgroup.
+template <typename T>
+struct OpHistogram {
+    static constexpr int MAXB = 512;
+    struct Shared {
+        unsigned int cnt[MAXB];
+        double vs[MAXB];
+    };
+    struct Lane {
+        T vx, vy, vz;
+    };
+    int nbins;
+    T width;
+    const T *vel;       // n*D input frame, may be null
+    T rot[9];           // maps input-frame velocities to the aligned frame
+    int tric;
+    int dim;
+    unsigned long long *counts;
+    double *vsum;
+
+    __device__ void wave_init(Shared &sh) {
+        for (int k = threadIdx.x; k < nbins; k += 64) { sh.cnt[k] = 0; sh.vs[k] = 0; }
+        __syncthreads();
+    }
+    __device__ void rotv(const T *v, T *o) const {
+        if (tric) {
+            for (int r = 0; r < 3; ++r) o[r] = (rot[r] * v[0] + rot[r + 3] * v[1]) + rot[r + 6] * v[2];
+        } else {
+            o[0] = v[0]; o[1] = v[1]; o[2] = v[2];
+        }
+    }
+    __device__ void lane_init(Lane &l, const Ent<T> &p, bool valid) {
+        l.vx = l.vy = l.vz = T(0);
+        if (vel && valid) {
+            const uint32_t i = ent_w(p) & IDX_MASK;
+            T v[3] = {T(0), T(0), T(0)}, o[3];
+            for (int k = 0; k < dim; ++k) v[k] = vel[(size_t)i * dim + k];
+            rotv(v, o);
+            l.vx = o[0]; l.vy = o[1]; l.vz = o[2];
+        }
+    }
+    __device__ void candidate(Shared &sh, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+        if (!ok) return;
+        const T d = dev_sqrt_rn(d2);
+        T bf = dev_floor(dev_div_rn(d, width));
+        int b = bf > T(nbins - 1) ? nbins - 1 : (int)bf;
+        atomicAdd(&sh.cnt[b], 1u);
+        if (vel) {
+            const uint32_t j = ent_w(q) & IDX_MASK;
+            T v[3] = {T(0), T(0), T(0)}, o[3];
+            for (int k = 0; k < dim; ++k) v[k] = vel[(size_t)j * dim + k];
+            rotv(v, o);
+            const T t = (l.vx - o[0]) * (p.x - q.x) + (l.vy - o[1]) * (p.y - q.y) + (l.vz - o[2]) * (p.z - q.z);
+            if (d > T(0)) atomicAdd(&sh.vs[b], (double)(t / d));
+        }
+    }
+    __device__ void lane_end(Shared &, Lane &, const Ent<T> &, bool) {}
+    __device__ void wave_end(Shared &sh) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < nbins; k += 64) {
+            if (sh.cnt[k]) atomicAdd(&counts[k], (unsigned long long)sh.cnt[k]);
+            if (vel && sh.vs[k] != 0.0) atomicAdd(&vsum[k], sh.vs[k]);
+        }
+    }
+};
+
+// Minimum distance: per lane best (d2, a, b); per workgroup result written to an array.
+template <typename T>
+struct OpMinDist {
+    struct Shared {
+        T d2;
+        int a, b;
+    };
+    struct Lane {
+        T d2;
+        int a, b;
+    };
+    T *wd2;      // [gridDim.x]
+    int *wa;     // [gridDim.x]
+    int *wb;     // [gridDim.x]
+
+    __host__ __device__ static bool better(T d2, int a, int b, T e2, int ea, int eb) {
+        return d2 < e2 || (d2 == e2 && (a < ea || (a == ea && b < eb)));
+    }
+    __device__ void wave_init(Shared &sh) {
+        if (threadIdx.x == 0) { sh.d2 = T(INFINITY); sh.a = sh.b = 0x7fffffff; }
+        __syncthreads();
+    }
+    __device__ void lane_init(Lane &l, const Ent<T> &, bool) {
+        l.d2 = T(INFINITY);
+        l.a = l.b = 0x7fffffff;
+    }
+    __device__ void candidate(Shared &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+        if (!ok) return;
+        int i = (int)(ent_w(p) & IDX_MASK), j = (int)(ent_w(q) & IDX_MASK);
+        int a = i < j ? i : j, b = i < j ? j : i;
+        if (better(d2, a, b, l.d2, l.a, l.b)) { l.d2 = d2; l.a = a; l.b = b; }
+    }
+    __device__ void lane_end(Shared &sh, Lane &l, const Ent<T> &, bool) {
+        T d2 = l.d2;
+        int a = l.a, b = l.b;
+        for (int off = 32; off > 0; off >>= 1) {
+            T d2o = __shfl_xor(d2, off, 64);
+            int ao = __shfl_xor(a, off, 64), bo = __shfl_xor(b, off, 64);
+            if (better(d2o, ao, bo, d2, a, b)) { d2 = d2o; a = ao; b = bo; }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && better(d2, a, b, sh.d2, sh.a, sh.b)) { sh.d2 = d2; sh.a = a; sh.b = b; }
+        __syncthreads();
+    }
+    __device__ void wave_end(Shared &sh) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            wd2[blockIdx.x] = sh.d2;
+            wa[blockIdx.x] = sh.a;
+            wb[blockIdx.x] = sh.b;
+        }
+    }
+};
+
+}  // namespace clm

@@ -1,0 +1,158 @@
+/*
+ * clm_hip.h -- C ABI of the MI355X (gfx950) cell-list pairwise engine.
+ *
+ * Plain C types only: opaque context, pointers, sizes, int status codes.
+ * Indices are 0-based int32.  Matrices are column-major dim x dim with the
+ * columns holding the lattice vectors (the reference's convention).
+ * Every entry point names the reference interface it replaces
+ * (paths relative to the CellListMap.jl source tree).
+ *
+ * Threading: one context = one HIP stream; a context must not be used from two
+ * host threads at once.  Calls return after their device work has finished
+ * unless stated otherwise.
+ *
+ * Ownership: host buffers are caller-owned and only touched during the call.
+ * Device buffers passed with on_device=1 are caller-owned device pointers that
+ * must stay valid until the call returns.  All scratch device memory is owned
+ * by the context and grows monotonically (InPlaceNeighborList semantics,
+ * src/API/neighborlist.jl:12-15, src/API/updating.jl:18-25).
+ */
+#ifndef CLM_HIP_H
+#define CLM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct clm_ctx clm_ctx;
+
+/* Status codes.  ARG/DIM/CELL/NAN map to the reference's ArgumentError /
+ * DimensionMismatch (src/internals/Box.jl:192-194,243; src/internals/CellOperations.jl:6-21;
+ * src/internals/CellLists.jl:740-751). */
+typedef enum {
+    CLM_OK = 0,
+    CLM_ERR_ARG = 1,        /* invalid argument (lcell < 1, bad dtype, NULL pointer, ...) */
+    CLM_ERR_DIM = 2,        /* dimension mismatch */
+    CLM_ERR_CELL = 3,       /* unit cell check failed: a plane distance <= 2*cutoff */
+    CLM_ERR_NAN = 4,        /* NaN or Inf coordinates */
+    CLM_ERR_CAPACITY = 5,   /* output buffer too small; the required size is returned */
+    CLM_ERR_HIP = 6,        /* HIP runtime error (see clm_last_error) */
+    CLM_ERR_STATE = 7,      /* call order (no box / no positions yet) */
+    CLM_ERR_RANGE = 8       /* a particle maps outside the computing grid (reference: BoundsError) */
+} clm_status;
+
+typedef enum { CLM_F32 = 0, CLM_F64 = 1 } clm_dtype;
+typedef enum { CLM_ORTHORHOMBIC = 0, CLM_TRICLINIC = 1, CLM_NONPERIODIC = 2 } clm_celltype;
+
+/* Box geometry as built by the library (values of the working precision). */
+typedef struct {
+    int dim, celltype, lcell;
+    int nc[3];                  /* computing cells per dimension (Box.nc) */
+    double cutoff, cutoff_sqr;
+    double input[9];            /* input unit cell (column-major 3x3 storage) */
+    double aligned[9];          /* aligned unit cell */
+    double rotation[9], inv_rotation[9];
+    double cell_size[3];
+    double cbmin[3], cbmax[3];  /* computing box */
+    double origin[3];
+} clm_box_info;
+
+/* Soft-core pair potential u(d2) = eps_ij[(s)^6 - (s)^3], s = sigma_ij^2/(d2+1).
+ * ntypes = 0: a single (eps[0], sigma[0]) for every pair.  ntypes > 0: per-type
+ * eps/sigma with Lorentz-Berthelot mixing, particle types from clm_set_types. */
+typedef struct {
+    int ntypes;
+    double eps[8];
+    double sigma[8];
+} clm_softcore;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* No reference counterpart (the reference has no device state). */
+clm_status clm_create(clm_ctx **ctx, int device, int dtype, int dim);
+clm_status clm_destroy(clm_ctx *ctx);
+const char *clm_last_error(const clm_ctx *ctx);
+/* Launch on an existing HIP stream (hipStream_t passed as void*); NULL = the context's own. */
+clm_status clm_set_stream(clm_ctx *ctx, void *stream);
+/* Library version string. */
+const char *clm_version(void);
+
+/* ---- geometry ----------------------------------------------------------- */
+
+/* Box(unitcell, cutoff; lcell) / update_box: src/internals/Box.jl:191-270,330-335,395-423.
+ * cell: dim*dim column-major (ORTHORHOMBIC takes the diagonal).  NONPERIODIC: cell is
+ * ignored and the box is rebuilt from limits(x) at each cell-list update
+ * (Box.jl:374-377; src/internals/ParticleSystem.jl:158-164). */
+clm_status clm_set_box(clm_ctx *ctx, const double *cell, int celltype, double cutoff, int lcell);
+clm_status clm_get_box(clm_ctx *ctx, clm_box_info *out);
+/* Host-only box construction (no device needed): the same arithmetic as clm_set_box. */
+clm_status clm_box_compute(int dtype, int dim, const double *cell, int celltype, double cutoff, int lcell,
+                           const double *origin, clm_box_info *out);
+
+/* ---- particles ---------------------------------------------------------- */
+
+/* update!(sys; xpositions=x): src/API/updating.jl:79-113,165-187 (copies, marks dirty).
+ * x: n*dim values of the context dtype, AoS (the memory of Vector{SVector{dim,T}}). */
+clm_status clm_set_positions(clm_ctx *ctx, const void *x, int64_t n, int on_device);
+/* Per-particle int32 types in [0, ntypes) for typed soft-core (no reference counterpart). */
+clm_status clm_set_types(clm_ctx *ctx, const int32_t *types, int64_t n, int on_device);
+/* Per-particle velocities (n*dim, context dtype) for the radial-velocity histogram. */
+clm_status clm_set_velocities(clm_ctx *ctx, const void *v, int64_t n, int on_device);
+
+/* UpdateCellList!(x, box, cl): src/internals/CellLists.jl:644-662,727-927.
+ * Operators call it lazily when positions or box changed (UpdateParticleSystem!). */
+clm_status clm_update_cell_list(clm_ctx *ctx);
+/* CellList fields (src/internals/CellLists.jl:115-137): out = {n_particles incl. images,
+ * n_cells_with_real_particles, n_cells_with_particles, number_of_cells}. */
+clm_status clm_cell_list_info(clm_ctx *ctx, int64_t *out4);
+
+/* ---- device pair operators (stand-ins for pairwise!(f, sys) with fixed f) -- */
+
+/* neighborlist!(system) / push_pair!: src/API/neighborlist.jl:217-231,
+ * src/internals/neighborlist.jl:67-76.  Writes (i[k], j[k], d[k]) for every pair
+ * within the cutoff, d in the context dtype, order unspecified.  If the pair count
+ * exceeds capacity, only capacity pairs are written, *npairs holds the required
+ * count and CLM_ERR_CAPACITY is returned. */
+clm_status clm_neighborlist(clm_ctx *ctx, int32_t *i, int32_t *j, void *d, int64_t capacity,
+                            int64_t *npairs, int on_device);
+
+/* pairwise!(f, sys) with f = soft-core energy and forces (F_i = -dE/dx_i, n*dim,
+ * context dtype, input frame).  forces may be NULL (energy only).  energy is
+ * accumulated in double.  Replaces the user closure of
+ * docs/src/ParticleSystem/examples.md:60-106 / test/examples/gravitational_force.jl:37-46. */
+clm_status clm_energy_forces(clm_ctx *ctx, const clm_softcore *p, double *energy, void *forces,
+                             int on_device, int64_t *npairs);
+
+/* Energy and its adjoint in one sweep: dE/dx (n*dim, context dtype), dE/deps and
+ * dE/dsigma per type (double, p->ntypes entries, or 1 when untyped).  Positions
+ * are re-read, no pair data is stored.  No reference counterpart (the reference
+ * differentiates closures with ForwardDiff, test/examples/generic_types.jl:25-44). */
+clm_status clm_energy_adjoint(clm_ctx *ctx, const clm_softcore *p, double *energy, void *dedx,
+                              double *deps, double *dsigma, int on_device);
+
+/* Distance histogram (test/examples/distance_histogram.jl:22-26) with bins of width
+ * `width` starting at 0 (bin = floor(d/width), the d == cutoff edge in the last
+ * bin), counts exact; with velocities set and vsum != NULL, also the sum per bin
+ * of dot(v_i - v_j, x_i - x_j)/d (test/examples/pairwise_velocities.jl:17-24). */
+clm_status clm_histogram(clm_ctx *ctx, int nbins, double width, int64_t *counts, double *vsum);
+
+/* Minimum distance over pairs within the cutoff (test/examples/nearest_neighbor.jl:9-16):
+ * lexicographic minimum of (d, min(i,j), max(i,j)); i = j = -1, d = +inf if none. */
+clm_status clm_min_distance(clm_ctx *ctx, int64_t *i, int64_t *j, double *d);
+
+/* ---- measurement ----------------------------------------------------------- */
+
+/* Per-kernel HIP-event timing on the context stream (no reference counterpart;
+ * the reference has ProgressMeter only, src/internals/self.jl:55,91). */
+clm_status clm_profile_enable(clm_ctx *ctx, int enable);
+/* Accumulated milliseconds and launch count of kernel `name` since enabling. */
+clm_status clm_profile_get(clm_ctx *ctx, const char *name, double *total_ms, int64_t *count);
+clm_status clm_profile_reset(clm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLM_HIP_H */

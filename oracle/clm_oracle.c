@@ -1,0 +1,651 @@
+/*
+ * clm_oracle.c -- CPU oracle for the cell-list pairwise engine.
+ *
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the CPU-baseline
+ * leg of bench.py are the only callers.  The HIP library never links this.
+ *
+ * It restates, in plain C, the algorithm of CellListMap.jl (reference
+ * src/internals/{Box,CellOperations,CellLists,self,vicinal_cells,
+ * auxiliary_functions,NonPeriodicCells}.jl) for one set of particles:
+ *   - Box construction (Box.jl:191-270), align_cell (CellOperations.jl:353-423),
+ *     check_unit_cell (Box.jl:579-636), cell_limits (CellOperations.jl:431-479);
+ *   - wrapping to the first cell + rotation (CellOperations.jl:56-94, CellLists.jl:944-945);
+ *   - cell assignment with the border rule and the 26 (8 in 2D) ghost images
+ *     (Box.jl:499-566, CellLists.jl:956-1057);
+ *   - the cell sweep (self.jl:106-184, vicinal_cells.jl:4-75, auxiliary_functions.jl:66-111),
+ *     with (mode 0) or without (mode 1) the projection early exit;
+ *   - the brute-force minimum-image loop of the reference's tests (mode 2,
+ *     test/modules/Testing.jl:76-89);
+ * plus the device operators' definitions (soft-core energy/forces and its
+ * parameter adjoint, distance histogram + mean radial velocity, minimum distance,
+ * neighbour list) evaluated on the pairs it finds, accumulated in double.
+ *
+ * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { ORC_ORTHO = 0, ORC_TRICLINIC = 1, ORC_NONPERIODIC = 2 };
+enum { ORC_OK = 0, ORC_ERR_LCELL = 1, ORC_ERR_CELL = 2, ORC_ERR_NAN = 3, ORC_ERR_RANGE = 4,
+       ORC_ERR_NOMEM = 5, ORC_ERR_DTYPE = 6, ORC_ERR_ARG = 7 };
+
+/* ---- double instantiation ---- */
+#define REAL double
+#define SFX(n) n##_f64
+#define RFLOOR floor
+#define RCEIL ceil
+#define RSQRT sqrt
+#define RFABS fabs
+#define RFMOD fmod
+#include "oracle_impl.h"
+#undef REAL
+#undef SFX
+#undef RFLOOR
+#undef RCEIL
+#undef RSQRT
+#undef RFABS
+#undef RFMOD
+
+/* ---- float instantiation ---- */
+#define REAL float
+#define SFX(n) n##_f32
+#define RFLOOR floorf
+#define RCEIL ceilf
+#define RSQRT sqrtf
+#define RFABS fabsf
+#define RFMOD fmodf
+#include "oracle_impl.h"
+#undef REAL
+#undef SFX
+#undef RFLOOR
+#undef RCEIL
+#undef RSQRT
+#undef RFABS
+#undef RFMOD
+
+/* ------------------------------------------------------------------ */
+/* public box description (values of the REAL type, stored as double)   */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int dtype;      /* 0 = float32, 1 = float64 */
+    int dim, celltype, lcell;
+    int nc[3];
+    double cutoff, cutoff_sqr;
+    double input[9], aligned[9], rot[9], invrot[9];
+    double cs[3], cbmin[3], cbmax[3], origin[3];
+} orc_box;
+
+#define COPY_BOX_OUT(bx, out)                                             \
+    do {                                                                  \
+        (out)->dim = (bx).dim; (out)->celltype = (bx).celltype;           \
+        (out)->lcell = (bx).lcell;                                        \
+        for (int k_ = 0; k_ < 3; ++k_) {                                  \
+            (out)->nc[k_] = (bx).nc[k_]; (out)->cs[k_] = (bx).cs[k_];     \
+            (out)->cbmin[k_] = (bx).cbmin[k_]; (out)->cbmax[k_] = (bx).cbmax[k_]; \
+            (out)->origin[k_] = (bx).origin[k_];                          \
+        }                                                                 \
+        for (int k_ = 0; k_ < 9; ++k_) {                                  \
+            (out)->input[k_] = (bx).input[k_]; (out)->aligned[k_] = (bx).aligned[k_]; \
+            (out)->rot[k_] = (bx).rot[k_]; (out)->invrot[k_] = (bx).invrot[k_]; \
+        }                                                                 \
+        (out)->cutoff = (bx).cutoff; (out)->cutoff_sqr = (bx).cutoff_sqr; \
+    } while (0)
+
+#define COPY_BOX_IN(in, bx, R)                                            \
+    do {                                                                  \
+        (bx).dim = (in)->dim; (bx).celltype = (in)->celltype;             \
+        (bx).lcell = (in)->lcell;                                         \
+        for (int k_ = 0; k_ < 3; ++k_) {                                  \
+            (bx).nc[k_] = (in)->nc[k_]; (bx).cs[k_] = (R)(in)->cs[k_];    \
+            (bx).cbmin[k_] = (R)(in)->cbmin[k_]; (bx).cbmax[k_] = (R)(in)->cbmax[k_]; \
+            (bx).origin[k_] = (R)(in)->origin[k_];                        \
+        }                                                                 \
+        for (int k_ = 0; k_ < 9; ++k_) {                                  \
+            (bx).input[k_] = (R)(in)->input[k_]; (bx).aligned[k_] = (R)(in)->aligned[k_]; \
+            (bx).rot[k_] = (R)(in)->rot[k_]; (bx).invrot[k_] = (R)(in)->invrot[k_]; \
+        }                                                                 \
+        (bx).cutoff = (R)(in)->cutoff; (bx).cutoff_sqr = (R)(in)->cutoff_sqr; \
+    } while (0)
+
+/*
+ * Box(unitcell, cutoff; lcell) for a periodic cell, or Box(limits(x), cutoff)
+ * for a non-periodic one (Box.jl:374-377: sides = limits + 2.1 rc, origin = xmin).
+ * cell: column-major dim x dim (columns = lattice vectors); for ORC_NONPERIODIC
+ * pass the diagonal sides already computed by orc_nonperiodic_sides.
+ */
+int orc_box_build(int dtype, int dim, const double *cell, int celltype, double cutoff, int lcell,
+                  const double *origin, orc_box *out) {
+    if (dim != 2 && dim != 3) return ORC_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    out->dtype = dtype;
+    if (dtype == 1) {
+        Box_f64 b;
+        double c[9], o[3] = {0, 0, 0};
+        for (int k = 0; k < dim * dim; ++k) c[k] = cell[k];
+        if (origin) for (int k = 0; k < dim; ++k) o[k] = origin[k];
+        int rc = box_build_f64(dim, c, celltype, cutoff, lcell, o, &b);
+        if (rc) return rc;
+        COPY_BOX_OUT(b, out);
+    } else if (dtype == 0) {
+        Box_f32 b;
+        float c[9], o[3] = {0, 0, 0};
+        for (int k = 0; k < dim * dim; ++k) c[k] = (float)cell[k];
+        if (origin) for (int k = 0; k < dim; ++k) o[k] = (float)origin[k];
+        int rc = box_build_f32(dim, c, celltype, (float)cutoff, lcell, o, &b);
+        if (rc) return rc;
+        COPY_BOX_OUT(b, out);
+    } else {
+        return ORC_ERR_DTYPE;
+    }
+    return 0;
+}
+
+/* limits(x) (CellOperations.jl:262-297) and _sides_from_limits (Box.jl:38) */
+int orc_nonperiodic_sides(int dtype, int dim, const void *x, int64_t n, double cutoff, double *sides, double *origin) {
+    for (int k = 0; k < dim; ++k) { sides[k] = 0; origin[k] = 0; }
+    if (dtype == 1) {
+        const double *xd = (const double *)x;
+        double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+        if (n > 0) {
+            for (int k = 0; k < dim; ++k) { mn[k] = INFINITY; mx[k] = -INFINITY; }
+            for (int64_t i = 0; i < n; ++i)
+                for (int k = 0; k < dim; ++k) {
+                    double v = xd[i * dim + k];
+                    if (v != v) return ORC_ERR_NAN;
+                    if (v < mn[k]) mn[k] = v;
+                    if (v > mx[k]) mx[k] = v;
+                }
+        }
+        double pad = 210 * cutoff / 100;
+        for (int k = 0; k < dim; ++k) { sides[k] = (mx[k] - mn[k]) + pad; origin[k] = mn[k]; }
+    } else {
+        const float *xf = (const float *)x;
+        float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+        if (n > 0) {
+            for (int k = 0; k < dim; ++k) { mn[k] = INFINITY; mx[k] = -INFINITY; }
+            for (int64_t i = 0; i < n; ++i)
+                for (int k = 0; k < dim; ++k) {
+                    float v = xf[i * dim + k];
+                    if (v != v) return ORC_ERR_NAN;
+                    if (v < mn[k]) mn[k] = v;
+                    if (v > mx[k]) mx[k] = v;
+                }
+        }
+        float c = (float)cutoff;
+        float pad = (210 * c) / 100;
+        for (int k = 0; k < dim; ++k) { sides[k] = (double)((mx[k] - mn[k]) + pad); origin[k] = mn[k]; }
+    }
+    return 0;
+}
+
+/* wrapped + rotated coordinates of each particle (what the cell list stores for reals) */
+int orc_wrap(const orc_box *in, const void *x, int64_t n, void *out) {
+    if (in->dtype == 1) {
+        Box_f64 b; COPY_BOX_IN(in, b, double);
+        const double *xd = (const double *)x; double *o = (double *)out;
+        for (int64_t i = 0; i < n; ++i) {
+            double xi[3] = {0, 0, 0}, p[3];
+            for (int k = 0; k < b.dim; ++k) xi[k] = xd[i * b.dim + k];
+            wrap_rotate_f64(&b, xi, p);
+            for (int k = 0; k < b.dim; ++k) o[i * b.dim + k] = p[k];
+        }
+    } else {
+        Box_f32 b; COPY_BOX_IN(in, b, float);
+        const float *xf = (const float *)x; float *o = (float *)out;
+        for (int64_t i = 0; i < n; ++i) {
+            float xi[3] = {0, 0, 0}, p[3];
+            for (int k = 0; k < b.dim; ++k) xi[k] = xf[i * b.dim + k];
+            wrap_rotate_f32(&b, xi, p);
+            for (int k = 0; k < b.dim; ++k) o[i * b.dim + k] = p[k];
+        }
+    }
+    return 0;
+}
+
+/* cell-list statistics: out = {n_entries incl. images, n cells with real particles,
+ * n cells with particles, total cells} (CellList fields, CellLists.jl:115-137) */
+int orc_cl_stats(const orc_box *in, const void *x, int64_t n, int64_t *out) {
+    int rc;
+    if (in->dtype == 1) {
+        Box_f64 b; COPY_BOX_IN(in, b, double);
+        CellList_f64 cl;
+        rc = cl_build_f64(&b, (const double *)x, n, &cl);
+        if (!rc) { out[0] = cl.n_entries; out[1] = cl.n_home; out[2] = cl.n_cells_with_particles; out[3] = cl.ncells; }
+        cl_free_f64(&cl);
+    } else {
+        Box_f32 b; COPY_BOX_IN(in, b, float);
+        CellList_f32 cl;
+        rc = cl_build_f32(&b, (const float *)x, n, &cl);
+        if (!rc) { out[0] = cl.n_entries; out[1] = cl.n_home; out[2] = cl.n_cells_with_particles; out[3] = cl.ncells; }
+        cl_free_f32(&cl);
+    }
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* pair collection                                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t n, cap;
+    int32_t *i, *j;
+    double *d2, *d;
+    double xv[3];
+} pair_sink;
+
+static void sink_f64(void *ctx, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
+    (void)xi; (void)xj;
+    pair_sink *s = (pair_sink *)ctx;
+    if (s->n < s->cap) {
+        s->i[s->n] = (int32_t)i; s->j[s->n] = (int32_t)j;
+        s->d2[s->n] = d2; s->d[s->n] = sqrt(d2);
+    }
+    s->n++;
+}
+
+static void sink_f32(void *ctx, int64_t i, int64_t j, const float *xi, const float *xj, float d2) {
+    (void)xi; (void)xj;
+    pair_sink *s = (pair_sink *)ctx;
+    if (s->n < s->cap) {
+        s->i[s->n] = (int32_t)i; s->j[s->n] = (int32_t)j;
+        s->d2[s->n] = (double)d2; s->d[s->n] = (double)sqrtf(d2);
+    }
+    s->n++;
+}
+
+/*
+ * All pairs within the cutoff (neighbour list, internals/neighborlist.jl:67-76).
+ * mode 0: reference sweep with projection filters; 1: same sweep without them;
+ * 2: brute-force minimum image.  d is sqrt(d2) in the working precision.
+ * Returns the number of pairs (entries beyond cap are counted, not written);
+ * a negative value is -(error code).
+ */
+int64_t orc_pairs(const orc_box *in, const void *x, int64_t n, int mode,
+                  int32_t *pi, int32_t *pj, double *pd2, double *pd, int64_t cap) {
+    pair_sink s = {0, cap, pi, pj, pd2, pd, {0, 0, 0}};
+    int rc = 0;
+    if (in->dtype == 1) {
+        Box_f64 b; COPY_BOX_IN(in, b, double);
+        if (mode == 2) {
+            rc = naive_f64(&b, (const double *)x, n, sink_f64, &s);
+        } else {
+            CellList_f64 cl;
+            rc = cl_build_f64(&b, (const double *)x, n, &cl);
+            if (!rc) rc = sweep_f64(&b, &cl, mode == 0, sink_f64, &s, 1, 0);
+            cl_free_f64(&cl);
+        }
+    } else {
+        Box_f32 b; COPY_BOX_IN(in, b, float);
+        if (mode == 2) {
+            rc = naive_f32(&b, (const float *)x, n, sink_f32, &s);
+        } else {
+            CellList_f32 cl;
+            rc = cl_build_f32(&b, (const float *)x, n, &cl);
+            if (!rc) rc = sweep_f32(&b, &cl, mode == 0, sink_f32, &s, 1, 0);
+            cl_free_f32(&cl);
+        }
+    }
+    return rc ? -(int64_t)rc : s.n;
+}
+
+/* ------------------------------------------------------------------ */
+/* soft-core energy, forces and the parameter adjoint (double)          */
+/* ------------------------------------------------------------------ */
+
+/*
+ * u(d2) = eps_ij [ s^6 - s^3 ],  s = sigma_ij^2 / (d2 + 1)
+ * Lorentz-Berthelot: sigma_ij = (sigma_i + sigma_j)/2, eps_ij = sqrt(eps_i eps_j).
+ * du/dd2 = -eps_ij (6 s^6 - 3 s^3) / (d2 + 1);  F_i = -2 du/dd2 (x_i - x_j).
+ * dE/d eps_t  += (s^6 - s^3) * eps_ij / (2 eps_t)   for t in {t_i, t_j}
+ * dE/d sigma_t += eps_ij (6 s^6 - 3 s^3) / sigma_ij  for t in {t_i, t_j}  (= 1/2 * 2 eps (6s^6-3s^3)/sigma)
+ * Pair vectors are mapped back to the input frame with inv_rotation, as NeighborPair
+ * does for triclinic cells (self.jl:171-178).
+ */
+typedef struct {
+    const int32_t *types;
+    const double *eps, *sig;
+    double energy;
+    double *forces;   /* n*dim, input frame */
+    double *deps, *dsig;
+    int dim;
+    const double *invrot;
+    int64_t npairs;
+} sc_ctx;
+
+static void sc_pair(void *c, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
+    sc_ctx *s = (sc_ctx *)c;
+    int ti = s->types ? s->types[i] : 0, tj = s->types ? s->types[j] : 0;
+    double e = sqrt(s->eps[ti] * s->eps[tj]);
+    double sg = (s->sig[ti] + s->sig[tj]) / 2;
+    double inv = 1.0 / (d2 + 1.0);
+    double sv = sg * sg * inv;
+    double s3 = sv * sv * sv, s6 = s3 * s3;
+    s->energy += e * (s6 - s3);
+    s->npairs++;
+    if (s->forces) {
+        double g = 2.0 * e * (6.0 * s6 - 3.0 * s3) * inv;   /* F_i = g (x_i - x_j) */
+        double r[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
+        for (int k = 0; k < s->dim; ++k) r[k] = xi[k] - xj[k];
+        matvec_f64(s->invrot, r, ro, s->dim);
+        for (int k = 0; k < s->dim; ++k) {
+            s->forces[i * s->dim + k] += g * ro[k];
+            s->forces[j * s->dim + k] -= g * ro[k];
+        }
+    }
+    if (s->deps) {
+        double ge = s6 - s3;
+        s->deps[ti] += ge * e / (2.0 * s->eps[ti]);
+        s->deps[tj] += ge * e / (2.0 * s->eps[tj]);
+    }
+    if (s->dsig) {
+        double gs = e * (6.0 * s6 - 3.0 * s3) / sg;
+        s->dsig[ti] += gs;
+        s->dsig[tj] += gs;
+    }
+}
+
+/*
+ * x: double coordinates (n*dim), box must be a float64 box.  types may be NULL
+ * (all particles of type 0).  forces/deps/dsig may be NULL.  mode as orc_pairs.
+ */
+int orc_softcore(const orc_box *in, const double *x, int64_t n, const int32_t *types, int ntypes,
+                 const double *eps, const double *sig, int mode, double *energy, double *forces,
+                 double *deps, double *dsig, int64_t *npairs) {
+    if (in->dtype != 1) return ORC_ERR_DTYPE;
+    Box_f64 b; COPY_BOX_IN(in, b, double);
+    sc_ctx s = {types, eps, sig, 0.0, forces, deps, dsig, b.dim, b.invrot, 0};
+    if (forces) memset(forces, 0, sizeof(double) * n * b.dim);
+    if (deps) memset(deps, 0, sizeof(double) * ntypes);
+    if (dsig) memset(dsig, 0, sizeof(double) * ntypes);
+    int rc;
+    if (mode == 2) {
+        rc = naive_f64(&b, x, n, sc_pair, &s);
+    } else {
+        CellList_f64 cl;
+        rc = cl_build_f64(&b, x, n, &cl);
+        if (!rc) rc = sweep_f64(&b, &cl, mode == 0, sc_pair, &s, 1, 0);
+        cl_free_f64(&cl);
+    }
+    *energy = s.energy;
+    if (npairs) *npairs = s.npairs;
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* distance histogram + mean radial velocity (working precision bins)   */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Bin of a pair: b = floor(d / width) in the working precision with d = sqrt(d2),
+ * clamped to nbins-1 (the d == cutoff edge).  Counts are exact integers.  With
+ * velocities, vsum[b] += dot(v_i - v_j, x_i - x_j) / d accumulated in double
+ * (test/examples/pairwise_velocities.jl:17-24; a pair at d == 0 adds 0).
+ */
+typedef struct {
+    int nbins;
+    double width;
+    int64_t *counts;
+    double *vsum;
+    const void *v;
+    int dim;
+    const double *invrot;
+} hist_ctx;
+
+static void hist_pair_f64(void *c, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
+    hist_ctx *h = (hist_ctx *)c;
+    double d = sqrt(d2);
+    long b = (long)floor(d / h->width);
+    if (b > h->nbins - 1) b = h->nbins - 1;
+    if (b < 0) b = 0;
+    h->counts[b]++;
+    if (h->v && h->vsum) {
+        const double *v = (const double *)h->v;
+        double r[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
+        for (int k = 0; k < h->dim; ++k) r[k] = xi[k] - xj[k];
+        matvec_f64(h->invrot, r, ro, h->dim);
+        double t = 0;
+        for (int k = 0; k < h->dim; ++k) t += (v[i * h->dim + k] - v[j * h->dim + k]) * ro[k];
+        if (d > 0) h->vsum[b] += t / d;
+    }
+}
+
+static void hist_pair_f32(void *c, int64_t i, int64_t j, const float *xi, const float *xj, float d2) {
+    hist_ctx *h = (hist_ctx *)c;
+    float d = sqrtf(d2);
+    float w = (float)h->width;
+    long b = (long)floorf(d / w);
+    if (b > h->nbins - 1) b = h->nbins - 1;
+    if (b < 0) b = 0;
+    h->counts[b]++;
+    if (h->v && h->vsum) {
+        const float *v = (const float *)h->v;
+        double r[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
+        for (int k = 0; k < h->dim; ++k) r[k] = (double)xi[k] - (double)xj[k];
+        matvec_f64(h->invrot, r, ro, h->dim);
+        double t = 0;
+        for (int k = 0; k < h->dim; ++k) t += ((double)v[i * h->dim + k] - (double)v[j * h->dim + k]) * ro[k];
+        if (d > 0) h->vsum[b] += t / (double)d;
+    }
+}
+
+int orc_histogram(const orc_box *in, const void *x, int64_t n, const void *v, int nbins, double width,
+                  int mode, int64_t *counts, double *vsum) {
+    memset(counts, 0, sizeof(int64_t) * nbins);
+    if (vsum) memset(vsum, 0, sizeof(double) * nbins);
+    hist_ctx h = {nbins, width, counts, vsum, v, in->dim, in->invrot};
+    int rc;
+    if (in->dtype == 1) {
+        Box_f64 b; COPY_BOX_IN(in, b, double);
+        if (mode == 2) rc = naive_f64(&b, (const double *)x, n, hist_pair_f64, &h);
+        else {
+            CellList_f64 cl;
+            rc = cl_build_f64(&b, (const double *)x, n, &cl);
+            if (!rc) rc = sweep_f64(&b, &cl, mode == 0, hist_pair_f64, &h, 1, 0);
+            cl_free_f64(&cl);
+        }
+    } else {
+        Box_f32 b; COPY_BOX_IN(in, b, float);
+        h.width = (double)(float)width;
+        if (mode == 2) rc = naive_f32(&b, (const float *)x, n, hist_pair_f32, &h);
+        else {
+            CellList_f32 cl;
+            rc = cl_build_f32(&b, (const float *)x, n, &cl);
+            if (!rc) rc = sweep_f32(&b, &cl, mode == 0, hist_pair_f32, &h, 1, 0);
+            cl_free_f32(&cl);
+        }
+    }
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* minimum distance: lexicographic min of (d, min(i,j), max(i,j))       */
+/* (test/examples/nearest_neighbor.jl:9-16; ties resolved by index)     */
+/* ------------------------------------------------------------------ */
+
+typedef struct { double d; int64_t i, j; } md_ctx;
+
+static void md_take(md_ctx *m, double d, int64_t i, int64_t j) {
+    int64_t a = i < j ? i : j, b = i < j ? j : i;
+    if (d < m->d || (d == m->d && (a < m->i || (a == m->i && b < m->j)))) { m->d = d; m->i = a; m->j = b; }
+}
+static void md_pair_f64(void *c, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
+    (void)xi; (void)xj; md_take((md_ctx *)c, sqrt(d2), i, j);
+}
+static void md_pair_f32(void *c, int64_t i, int64_t j, const float *xi, const float *xj, float d2) {
+    (void)xi; (void)xj; md_take((md_ctx *)c, (double)sqrtf(d2), i, j);
+}
+
+int orc_mindist(const orc_box *in, const void *x, int64_t n, int mode, int64_t *oi, int64_t *oj, double *od) {
+    md_ctx m = {INFINITY, -1, -1};
+    int rc;
+    if (in->dtype == 1) {
+        Box_f64 b; COPY_BOX_IN(in, b, double);
+        if (mode == 2) rc = naive_f64(&b, (const double *)x, n, md_pair_f64, &m);
+        else {
+            CellList_f64 cl;
+            rc = cl_build_f64(&b, (const double *)x, n, &cl);
+            if (!rc) rc = sweep_f64(&b, &cl, mode == 0, md_pair_f64, &m, 1, 0);
+            cl_free_f64(&cl);
+        }
+    } else {
+        Box_f32 b; COPY_BOX_IN(in, b, float);
+        if (mode == 2) rc = naive_f32(&b, (const float *)x, n, md_pair_f32, &m);
+        else {
+            CellList_f32 cl;
+            rc = cl_build_f32(&b, (const float *)x, n, &cl);
+            if (!rc) rc = sweep_f32(&b, &cl, mode == 0, md_pair_f32, &m, 1, 0);
+            cl_free_f32(&cl);
+        }
+    }
+    *oi = m.i; *oj = m.j; *od = m.d;
+    return rc;
+}
+
+/* generic reductions used by the golden-vector tests: sum(d2), sum(d), sum(1/d), count */
+typedef struct { double sd2, sd, sinv; int64_t n; } red_ctx;
+static void red_f64(void *c, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
+    (void)i; (void)j; (void)xi; (void)xj;
+    red_ctx *r = (red_ctx *)c; double d = sqrt(d2);
+    r->sd2 += d2; r->sd += d; r->sinv += 1.0 / d; r->n++;
+}
+static void red_f32(void *c, int64_t i, int64_t j, const float *xi, const float *xj, float d2) {
+    (void)i; (void)j; (void)xi; (void)xj;
+    red_ctx *r = (red_ctx *)c; float d = sqrtf(d2);
+    r->sd2 += d2; r->sd += d; r->sinv += 1.0 / (double)d; r->n++;
+}
+
+int orc_reduce(const orc_box *in, const void *x, int64_t n, int mode, double *out4) {
+    red_ctx r = {0, 0, 0, 0};
+    int rc;
+    if (in->dtype == 1) {
+        Box_f64 b; COPY_BOX_IN(in, b, double);
+        if (mode == 2) rc = naive_f64(&b, (const double *)x, n, red_f64, &r);
+        else {
+            CellList_f64 cl;
+            rc = cl_build_f64(&b, (const double *)x, n, &cl);
+            if (!rc) rc = sweep_f64(&b, &cl, mode == 0, red_f64, &r, 1, 0);
+            cl_free_f64(&cl);
+        }
+    } else {
+        Box_f32 b; COPY_BOX_IN(in, b, float);
+        if (mode == 2) rc = naive_f32(&b, (const float *)x, n, red_f32, &r);
+        else {
+            CellList_f32 cl;
+            rc = cl_build_f32(&b, (const float *)x, n, &cl);
+            if (!rc) rc = sweep_f32(&b, &cl, mode == 0, red_f32, &r, 1, 0);
+            cl_free_f32(&cl);
+        }
+    }
+    out4[0] = r.sd2; out4[1] = r.sd; out4[2] = r.sinv; out4[3] = (double)r.n;
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* threaded CPU timing of the reference sweep on a sample of home cells */
+/* ------------------------------------------------------------------ */
+
+typedef struct { double e; int64_t n; double cs; } bench_acc;
+static void bench_sc_f32(void *c, int64_t i, int64_t j, const float *xi, const float *xj, float d2) {
+    (void)i; (void)j; (void)xi; (void)xj;
+    bench_acc *a = (bench_acc *)c;
+    float inv = 1.0f / (d2 + 1.0f);
+    float sv = 9.0f * inv, s3 = sv * sv * sv;
+    a->e += 0.2f * (s3 * s3 - s3);
+    a->n++;
+}
+static void bench_sc_f64(void *c, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
+    (void)i; (void)j; (void)xi; (void)xj;
+    bench_acc *a = (bench_acc *)c;
+    double inv = 1.0 / (d2 + 1.0);
+    double sv = 9.0 * inv, s3 = sv * sv * sv;
+    a->e += 0.2 * (s3 * s3 - s3);
+    a->n++;
+}
+
+/*
+ * Times the reference sweep (projection filters on, soft-core energy sigma=3,
+ * eps=0.2 per pair) over home cells h = offset, offset+stride, ... with
+ * nthreads OpenMP threads (dynamic schedule, one accumulator per thread), after
+ * building the cell list once.  out = {pairs, energy, sweep seconds, build seconds,
+ * home cells swept, total home cells}.
+ */
+int orc_bench_sweep(const orc_box *in, const void *x, int64_t n, int64_t stride, int nthreads, double *out) {
+    int rc = 0;
+    double t0 = 0, t1 = 0, t2 = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    t0 = omp_get_wtime();
+#endif
+    if (stride < 1) stride = 1;
+    if (in->dtype == 1) {
+        Box_f64 b; COPY_BOX_IN(in, b, double);
+        CellList_f64 cl;
+        rc = cl_build_f64(&b, (const double *)x, n, &cl);
+        if (rc) return rc;
+#ifdef _OPENMP
+        t1 = omp_get_wtime();
+#endif
+        int64_t nh = (cl.n_home + stride - 1) / stride;
+        double e = 0; int64_t np = 0;
+        int64_t maxn = 0;
+        for (int64_t c = 0; c < cl.ncells; ++c) { int64_t k = cl.cell_start[c + 1] - cl.cell_start[c]; if (k > maxn) maxn = k; }
+#pragma omp parallel reduction(+ : e, np)
+        {
+            double *px = (double *)malloc(sizeof(double) * (maxn + 1));
+            int64_t *pi = (int64_t *)malloc(sizeof(int64_t) * (maxn + 1));
+            bench_acc a = {0, 0, 0};
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t h = 0; h < nh; ++h) sweep_cell_f64(&b, &cl, cl.home[h * stride], 1, bench_sc_f64, &a, px, pi);
+            e += a.e; np += a.n;
+            free(px); free(pi);
+        }
+#ifdef _OPENMP
+        t2 = omp_get_wtime();
+#endif
+        out[0] = (double)np; out[1] = e; out[4] = (double)nh; out[5] = (double)cl.n_home;
+        cl_free_f64(&cl);
+    } else {
+        Box_f32 b; COPY_BOX_IN(in, b, float);
+        CellList_f32 cl;
+        rc = cl_build_f32(&b, (const float *)x, n, &cl);
+        if (rc) return rc;
+#ifdef _OPENMP
+        t1 = omp_get_wtime();
+#endif
+        int64_t nh = (cl.n_home + stride - 1) / stride;
+        double e = 0; int64_t np = 0;
+        int64_t maxn = 0;
+        for (int64_t c = 0; c < cl.ncells; ++c) { int64_t k = cl.cell_start[c + 1] - cl.cell_start[c]; if (k > maxn) maxn = k; }
+#pragma omp parallel reduction(+ : e, np)
+        {
+            float *px = (float *)malloc(sizeof(float) * (maxn + 1));
+            int64_t *pi = (int64_t *)malloc(sizeof(int64_t) * (maxn + 1));
+            bench_acc a = {0, 0, 0};
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t h = 0; h < nh; ++h) sweep_cell_f32(&b, &cl, cl.home[h * stride], 1, bench_sc_f32, &a, px, pi);
+            e += a.e; np += a.n;
+            free(px); free(pi);
+        }
+#ifdef _OPENMP
+        t2 = omp_get_wtime();
+#endif
+        out[0] = (double)np; out[1] = e; out[4] = (double)nh; out[5] = (double)cl.n_home;
+        cl_free_f32(&cl);
+    }
+    out[2] = t2 - t1; out[3] = t1 - t0;
+    return rc;
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,243 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and
+by the CPU-baseline leg of ``bench.py``, always as the checker of the HIP library,
+never as the thing measured or shipped.
+
+The oracle restates CellListMap.jl's single-set engine (see clm_oracle.c for the
+reference file:line of every step).  Indices are 0-based.  ``mode`` selects the
+pair source: 0 = reference sweep with projection filters, 1 = the same sweep
+without them, 2 = brute-force minimum image (test/modules/Testing.jl:76-89).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ORTHO, TRICLINIC, NONPERIODIC = 0, 1, 2
+ERRORS = {1: "lcell must be >= 1", 2: "unit cell check failed (plane distance <= 2*cutoff)",
+          3: "invalid coordinates (NaN/Inf)", 4: "cell index out of range",
+          5: "out of memory", 6: "dtype not supported here", 7: "bad argument"}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code):
+        super().__init__(f"oracle error {code}: {ERRORS.get(code, '?')}")
+        self.code = code
+
+
+class Box(C.Structure):
+    _fields_ = [("dtype", C.c_int), ("dim", C.c_int), ("celltype", C.c_int), ("lcell", C.c_int),
+                ("nc", C.c_int * 3), ("cutoff", C.c_double), ("cutoff_sqr", C.c_double),
+                ("input", C.c_double * 9), ("aligned", C.c_double * 9), ("rot", C.c_double * 9),
+                ("invrot", C.c_double * 9), ("cs", C.c_double * 3), ("cbmin", C.c_double * 3),
+                ("cbmax", C.c_double * 3), ("origin", C.c_double * 3)]
+
+    def info(self):
+        d = self.dim
+        m = lambda a: np.array(a[:]).reshape(3, 3, order="F")[:d, :d]  # noqa: E731
+        return dict(dim=d, celltype=self.celltype, lcell=self.lcell, nc=list(self.nc[:d]),
+                    cutoff=self.cutoff, cutoff_sqr=self.cutoff_sqr, cell_size=list(self.cs[:d]),
+                    cbmin=list(self.cbmin[:d]), cbmax=list(self.cbmax[:d]), aligned=m(self.aligned),
+                    rotation=m(self.rot), inv_rotation=m(self.invrot), input=m(self.input))
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile (gcc)."""
+    so = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        L = C.CDLL(so)
+        P, I64, D, I = C.c_void_p, C.c_int64, C.c_double, C.c_int
+        L.orc_box_build.argtypes = [I, I, P, I, D, I, P, C.POINTER(Box)]
+        L.orc_nonperiodic_sides.argtypes = [I, I, P, I64, D, P, P]
+        L.orc_wrap.argtypes = [C.POINTER(Box), P, I64, P]
+        L.orc_cl_stats.argtypes = [C.POINTER(Box), P, I64, P]
+        L.orc_pairs.argtypes = [C.POINTER(Box), P, I64, I, P, P, P, P, I64]
+        L.orc_pairs.restype = I64
+        L.orc_softcore.argtypes = [C.POINTER(Box), P, I64, P, I, P, P, I, P, P, P, P, P]
+        L.orc_histogram.argtypes = [C.POINTER(Box), P, I64, P, I, D, I, P, P]
+        L.orc_mindist.argtypes = [C.POINTER(Box), P, I64, I, P, P, P]
+        L.orc_reduce.argtypes = [C.POINTER(Box), P, I64, I, P]
+        L.orc_bench_sweep.argtypes = [C.POINTER(Box), P, I64, I64, I, P]
+        L.orc_max_threads.restype = I
+        _LIB = L
+    return _LIB
+
+
+def _np_dtype(dtype):
+    return np.float32 if dtype in (0, "f32", np.float32) else np.float64
+
+
+def _code(dtype):
+    return 0 if _np_dtype(dtype) == np.float32 else 1
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def make_box(unitcell, cutoff, lcell=1, dtype=np.float64, celltype=None, origin=None) -> Box:
+    """Box(unitcell, cutoff; lcell) (Box.jl:191-203,330-335).  A vector of sides gives an
+    orthorhombic cell; a matrix (columns = lattice vectors) a triclinic one."""
+    uc = np.asarray(unitcell, dtype=np.float64)
+    tdt = _np_dtype(dtype)
+    if uc.ndim == 1:
+        dim = uc.shape[0]
+        m = np.diag(uc.astype(tdt).astype(np.float64))
+        ct = ORTHO if celltype is None else celltype
+    else:
+        dim = uc.shape[0]
+        m = uc.astype(tdt).astype(np.float64)
+        ct = TRICLINIC if celltype is None else celltype
+    cell = np.ascontiguousarray(m.T.reshape(-1))  # column-major flattening
+    b = Box()
+    o = None if origin is None else np.ascontiguousarray(np.asarray(origin, dtype=np.float64))
+    rc = lib().orc_box_build(_code(dtype), dim, _ptr(cell), ct, float(tdt(cutoff)), int(lcell), _ptr(o), C.byref(b))
+    if rc:
+        raise OracleError(rc)
+    return b
+
+
+def make_nonperiodic_box(x, cutoff, lcell=1, dtype=np.float64) -> Box:
+    """Box(limits(x), cutoff) (Box.jl:374-377, CellOperations.jl:290-297)."""
+    x = np.ascontiguousarray(x, dtype=_np_dtype(dtype))
+    dim = x.shape[1]
+    sides = np.zeros(3)
+    origin = np.zeros(3)
+    rc = lib().orc_nonperiodic_sides(_code(dtype), dim, _ptr(x), x.shape[0], float(cutoff), _ptr(sides), _ptr(origin))
+    if rc:
+        raise OracleError(rc)
+    return make_box(sides[:dim], cutoff, lcell, dtype, celltype=NONPERIODIC, origin=origin[:dim])
+
+
+def _x(box, x):
+    return np.ascontiguousarray(np.asarray(x).reshape(-1, box.dim), dtype=np.float32 if box.dtype == 0 else np.float64)
+
+
+def wrap(box: Box, x):
+    x = _x(box, x)
+    out = np.empty_like(x)
+    lib().orc_wrap(C.byref(box), _ptr(x), x.shape[0], _ptr(out))
+    return out
+
+
+def cl_stats(box: Box, x):
+    x = _x(box, x)
+    out = np.zeros(4, dtype=np.int64)
+    rc = lib().orc_cl_stats(C.byref(box), _ptr(x), x.shape[0], _ptr(out))
+    if rc:
+        raise OracleError(rc)
+    return dict(n_particles=int(out[0]), n_cells_with_real_particles=int(out[1]),
+                n_cells_with_particles=int(out[2]), number_of_cells=int(out[3]))
+
+
+def pairs(box: Box, x, mode=0):
+    """(i, j, d2, d) arrays of every pair within the cutoff."""
+    x = _x(box, x)
+    cap = max(1024, x.shape[0] * 64)
+    while True:
+        i = np.empty(cap, np.int32); j = np.empty(cap, np.int32)
+        d2 = np.empty(cap, np.float64); d = np.empty(cap, np.float64)
+        n = lib().orc_pairs(C.byref(box), _ptr(x), x.shape[0], mode, _ptr(i), _ptr(j), _ptr(d2), _ptr(d), cap)
+        if n < 0:
+            raise OracleError(-n)
+        if n <= cap:
+            return i[:n], j[:n], d2[:n], d[:n]
+        cap = int(n)
+
+
+def canonical_pairs(i, j):
+    """Sorted (min, max) index pairs as one int64 key array (order-free comparison)."""
+    a = np.minimum(i, j).astype(np.int64)
+    b = np.maximum(i, j).astype(np.int64)
+    k = a * (1 << 32) + b
+    order = np.argsort(k, kind="stable")
+    return k[order], order
+
+
+def softcore(box: Box, x, eps=0.2, sigma=3.0, types=None, mode=0, forces=True, grads=False):
+    """FP64 soft-core energy (and forces, parameter gradients).  box must be float64."""
+    x = _x(box, x)
+    n = x.shape[0]
+    eps = np.atleast_1d(np.asarray(eps, np.float64))
+    sig = np.atleast_1d(np.asarray(sigma, np.float64))
+    nt = eps.shape[0]
+    t = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
+    e = np.zeros(1)
+    f = np.zeros((n, box.dim)) if forces else None
+    de = np.zeros(nt) if grads else None
+    ds = np.zeros(nt) if grads else None
+    npairs = np.zeros(1, np.int64)
+    rc = lib().orc_softcore(C.byref(box), _ptr(x), n, _ptr(t), nt, _ptr(eps), _ptr(sig), mode,
+                            _ptr(e), _ptr(f), _ptr(de), _ptr(ds), _ptr(npairs))
+    if rc:
+        raise OracleError(rc)
+    out = dict(energy=float(e[0]), npairs=int(npairs[0]))
+    if forces:
+        out["forces"] = f
+    if grads:
+        out["deps"] = de
+        out["dsigma"] = ds
+    return out
+
+
+def histogram(box: Box, x, nbins, width, v=None, mode=0):
+    x = _x(box, x)
+    vv = None if v is None else _x(box, v)
+    counts = np.zeros(nbins, np.int64)
+    vsum = np.zeros(nbins, np.float64)
+    rc = lib().orc_histogram(C.byref(box), _ptr(x), x.shape[0], _ptr(vv), int(nbins), float(width), mode,
+                             _ptr(counts), _ptr(vsum) if v is not None else None)
+    if rc:
+        raise OracleError(rc)
+    return counts, (vsum if v is not None else None)
+
+
+def mindist(box: Box, x, mode=0):
+    x = _x(box, x)
+    oi = np.zeros(1, np.int64); oj = np.zeros(1, np.int64); od = np.zeros(1)
+    rc = lib().orc_mindist(C.byref(box), _ptr(x), x.shape[0], mode, _ptr(oi), _ptr(oj), _ptr(od))
+    if rc:
+        raise OracleError(rc)
+    return int(oi[0]), int(oj[0]), float(od[0])
+
+
+def reduce(box: Box, x, mode=0):
+    """dict(sum_d2, sum_d, sum_invd, n) over all pairs within the cutoff."""
+    x = _x(box, x)
+    out = np.zeros(4)
+    rc = lib().orc_reduce(C.byref(box), _ptr(x), x.shape[0], mode, _ptr(out))
+    if rc:
+        raise OracleError(rc)
+    return dict(sum_d2=out[0], sum_d=out[1], sum_invd=out[2], n=int(out[3]))
+
+
+def bench_sweep(box: Box, x, stride=1, nthreads=0):
+    """Threaded timing of the reference sweep (soft-core energy) over every stride-th home cell."""
+    x = _x(box, x)
+    out = np.zeros(6)
+    rc = lib().orc_bench_sweep(C.byref(box), _ptr(x), x.shape[0], int(stride), int(nthreads), _ptr(out))
+    if rc:
+        raise OracleError(rc)
+    return dict(pairs=int(out[0]), energy=out[1], sweep_s=out[2], build_s=out[3],
+                home_swept=int(out[4]), home_total=int(out[5]))
+
+
+def max_threads():
+    return lib().orc_max_threads()

@@ -1,0 +1,688 @@
+/*
+ * oracle_impl.h -- CPU restatement of CellListMap.jl's periodic / non-periodic
+ * single-set cell-list engine, written once and instantiated for float and
+ * double (see clm_oracle.c, which defines REAL, SFX(), RFLOOR, RSQRT).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker for the HIP library;
+ * nothing in the product path may link or call it.
+ *
+ * Every function names the reference file:line whose arithmetic it restates.
+ * Operation order follows the Julia/StaticArrays expressions as read from the
+ * reference source (left-to-right sums, no fused multiply-add); the library's
+ * host geometry and device binning use the same order, so the two agree bit
+ * for bit.  Whether StaticArrays itself evaluates these expressions in exactly
+ * this order could not be checked here (no Julia toolchain): that ulp-level
+ * agreement with the Julia reference is "parity unpinned" and is stated as
+ * such in the repository's design notes.
+ */
+
+#define DEF(name) SFX(name)
+
+typedef struct {
+    int dim;          /* 2 or 3 */
+    int celltype;     /* ORC_ORTHO / ORC_TRICLINIC / ORC_NONPERIODIC */
+    int lcell;
+    int nc[3];        /* number of computing cells per dimension */
+    REAL cutoff, cutoff_sqr;
+    REAL input[9];    /* column-major 3x3 storage (m[i + 3*j]); 2D uses the top-left block */
+    REAL aligned[9];
+    REAL rot[9];
+    REAL invrot[9];
+    REAL cs[3];       /* computing cell sizes */
+    REAL cbmin[3], cbmax[3];
+    REAL origin[3];
+} DEF(Box);
+
+#define M3(m, i, j) ((m)[(i) + 3 * (j)])
+
+/* ------------------------------------------------------------------ */
+/* small linear algebra in the StaticArrays evaluation order           */
+/* ------------------------------------------------------------------ */
+
+/* sum(abs2, v) for an N-vector: ((v1^2 + v2^2) + v3^2) */
+static REAL DEF(sumsq)(const REAL *v, int n) {
+    REAL s = v[0] * v[0];
+    for (int k = 1; k < n; ++k) s = s + v[k] * v[k];
+    return s;
+}
+
+/* dot(a, b): ((a1 b1 + a2 b2) + a3 b3) */
+static REAL DEF(dot)(const REAL *a, const REAL *b, int n) {
+    REAL s = a[0] * b[0];
+    for (int k = 1; k < n; ++k) s = s + a[k] * b[k];
+    return s;
+}
+
+/* m * v for an n x n column-major matrix: row i = ((m[i,1]v1 + m[i,2]v2) + m[i,3]v3) */
+static void DEF(matvec)(const REAL *m, const REAL *v, REAL *out, int n) {
+    REAL t[3];
+    for (int i = 0; i < n; ++i) {
+        REAL s = M3(m, i, 0) * v[0];
+        for (int k = 1; k < n; ++k) s = s + M3(m, i, k) * v[k];
+        t[i] = s;
+    }
+    for (int i = 0; i < n; ++i) out[i] = t[i];
+}
+
+/* c = a * b, n x n, element (i,j) = ((a[i,1]b[1,j] + a[i,2]b[2,j]) + a[i,3]b[3,j]) */
+static void DEF(matmul)(const REAL *a, const REAL *b, REAL *c, int n) {
+    REAL t[9];
+    for (int i = 0; i < 9; ++i) t[i] = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            REAL s = M3(a, i, 0) * M3(b, 0, j);
+            for (int k = 1; k < n; ++k) s = s + M3(a, i, k) * M3(b, k, j);
+            M3(t, i, j) = s;
+        }
+    for (int i = 0; i < 9; ++i) c[i] = t[i];
+}
+
+static void DEF(identity)(REAL *m) {
+    for (int i = 0; i < 9; ++i) m[i] = 0;
+    m[0] = 1; m[4] = 1; m[8] = 1;
+}
+
+static void DEF(cross)(const REAL *a, const REAL *b, REAL *c) {
+    REAL t0 = a[1] * b[2] - a[2] * b[1];
+    REAL t1 = a[2] * b[0] - a[0] * b[2];
+    REAL t2 = a[0] * b[1] - a[1] * b[0];
+    c[0] = t0; c[1] = t1; c[2] = t2;
+}
+
+/*
+ * m \ x.  3x3: Cramer's rule with det = c1 . (c2 x c3) (StaticArrays solve/det
+ * for Size(3,3), third-party dependency StaticArrays >= 1.9.15, Project.toml:37).
+ * 2x2: ((a22 b1 - a12 b2)/d, (a11 b2 - a21 b1)/d) with d = a11 a22 - a12 a21.
+ */
+static void DEF(solve)(const REAL *a, const REAL *b, REAL *p, int n) {
+    if (n == 3) {
+        REAL c1[3] = {M3(a, 0, 0), M3(a, 1, 0), M3(a, 2, 0)};
+        REAL c2[3] = {M3(a, 0, 1), M3(a, 1, 1), M3(a, 2, 1)};
+        REAL c3[3] = {M3(a, 0, 2), M3(a, 1, 2), M3(a, 2, 2)};
+        REAL cc[3];
+        DEF(cross)(c2, c3, cc);
+        REAL d = DEF(dot)(c1, cc, 3);
+        REAL a11 = M3(a, 0, 0), a12 = M3(a, 0, 1), a13 = M3(a, 0, 2);
+        REAL a21 = M3(a, 1, 0), a22 = M3(a, 1, 1), a23 = M3(a, 1, 2);
+        REAL a31 = M3(a, 2, 0), a32 = M3(a, 2, 1), a33 = M3(a, 2, 2);
+        REAL p0 = (((a22 * a33 - a23 * a32) * b[0] + (a13 * a32 - a12 * a33) * b[1]) +
+                   (a12 * a23 - a13 * a22) * b[2]) / d;
+        REAL p1 = (((a23 * a31 - a21 * a33) * b[0] + (a11 * a33 - a13 * a31) * b[1]) +
+                   (a13 * a21 - a11 * a23) * b[2]) / d;
+        REAL p2 = (((a21 * a32 - a22 * a31) * b[0] + (a12 * a31 - a11 * a32) * b[1]) +
+                   (a11 * a22 - a12 * a21) * b[2]) / d;
+        p[0] = p0; p[1] = p1; p[2] = p2;
+    } else {
+        REAL a11 = M3(a, 0, 0), a12 = M3(a, 0, 1), a21 = M3(a, 1, 0), a22 = M3(a, 1, 1);
+        REAL d = a11 * a22 - a12 * a21;
+        REAL p0 = (a22 * b[0] - a12 * b[1]) / d;
+        REAL p1 = (a11 * b[1] - a21 * b[0]) / d;
+        p[0] = p0; p[1] = p1;
+    }
+}
+
+/* inverse of a rotation matrix (only used to map output vectors back; tolerance-level) */
+static void DEF(inverse)(const REAL *m, REAL *inv, int n) {
+    DEF(identity)(inv);
+    if (n == 3) {
+        REAL c1[3] = {M3(m, 0, 0), M3(m, 1, 0), M3(m, 2, 0)};
+        REAL c2[3] = {M3(m, 0, 1), M3(m, 1, 1), M3(m, 2, 1)};
+        REAL c3[3] = {M3(m, 0, 2), M3(m, 1, 2), M3(m, 2, 2)};
+        REAL y0[3], y1[3], y2[3];
+        DEF(cross)(c2, c3, y0);
+        REAL d = DEF(dot)(c1, y0, 3);
+        DEF(cross)(c3, c1, y1);
+        DEF(cross)(c1, c2, y2);
+        for (int k = 0; k < 3; ++k) {
+            M3(inv, 0, k) = y0[k] / d;
+            M3(inv, 1, k) = y1[k] / d;
+            M3(inv, 2, k) = y2[k] / d;
+        }
+    } else {
+        REAL a = M3(m, 0, 0), b = M3(m, 0, 1), c = M3(m, 1, 0), e = M3(m, 1, 1);
+        REAL d = a * e - b * c;
+        M3(inv, 0, 0) = e / d; M3(inv, 0, 1) = -b / d;
+        M3(inv, 1, 0) = -c / d; M3(inv, 1, 1) = a / d;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Box construction                                                     */
+/* ------------------------------------------------------------------ */
+
+/* align_cell for 2D: CellOperations.jl:353-375 */
+static void DEF(align_cell2d)(const REAL *min, REAL *mout, REAL *R) {
+    REAL a[2] = {M3(min, 0, 0), M3(min, 1, 0)};
+    REAL b[2] = {M3(min, 0, 1), M3(min, 1, 1)};
+    REAL na = RSQRT(DEF(sumsq)(a, 2));
+    REAL nb = RSQRT(DEF(sumsq)(b, 2));
+    if (nb > na) { a[0] = b[0]; a[1] = b[1]; na = nb; }
+    DEF(identity)(R);
+    for (int i = 0; i < 9; ++i) mout[i] = min[i];
+    if (a[1] == 0) return;
+    REAL sint = (-na) / ((a[0] * a[0]) / a[1] + a[1]);
+    REAL cost = ((-a[0]) * sint) / a[1];
+    M3(R, 0, 0) = cost; M3(R, 0, 1) = -sint;
+    M3(R, 1, 0) = sint; M3(R, 1, 1) = cost;
+    DEF(matmul)(R, min, mout, 2);
+}
+
+/* align_cell for 3D: CellOperations.jl:377-423 (longest vector on x, Rodrigues
+ * rotation R1, then a rotation about x built from column 2 of R1*m). */
+static void DEF(align_cell3d)(const REAL *min, REAL *mout, REAL *R) {
+    static const int comb[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    REAL n[3];
+    for (int k = 0; k < 3; ++k) {
+        REAL col[3] = {M3(min, 0, k), M3(min, 1, k), M3(min, 2, k)};
+        n[k] = DEF(sumsq)(col, 3);
+    }
+    int ia = 0;
+    for (int c = 0; c < 6; ++c) {
+        if (n[comb[c][0]] >= n[comb[c][1]] && n[comb[c][1]] >= n[comb[c][2]]) { ia = comb[c][0]; break; }
+    }
+    REAL a[3] = {M3(min, 0, ia), M3(min, 1, ia), M3(min, 2, ia)};
+    REAL na = RSQRT(DEF(sumsq)(a, 3));
+    REAL a1[3] = {a[0] / na, a[1] / na, a[2] / na};
+    REAL v[3] = {0, a1[2], -a1[1]};
+    REAL R1[9];
+    if (DEF(sumsq)(v, 3) == 0) {
+        DEF(identity)(R1);
+    } else {
+        REAL vs[9];
+        M3(vs, 0, 0) = 0;     M3(vs, 0, 1) = -v[2]; M3(vs, 0, 2) = v[1];
+        M3(vs, 1, 0) = v[2];  M3(vs, 1, 1) = 0;     M3(vs, 1, 2) = -v[0];
+        M3(vs, 2, 0) = -v[1]; M3(vs, 2, 1) = v[0];  M3(vs, 2, 2) = 0;
+        REAL vs2[9];
+        DEF(matmul)(vs, vs, vs2, 3);
+        REAL s = (REAL)1 / ((REAL)1 + a1[0]);
+        REAL I[9];
+        DEF(identity)(I);
+        for (int k = 0; k < 9; ++k) R1[k] = (I[k] + vs[k]) + vs2[k] * s;
+    }
+    REAL m[9];
+    DEF(matmul)(R1, min, m, 3);
+    REAL x = M3(m, 0, 1), y = M3(m, 1, 1), z = M3(m, 2, 1);
+    REAL R2[9];
+    DEF(identity)(R2);
+    if (!((y * y + z * z) == 0)) {
+        REAL col[3] = {x, y, z};
+        REAL b = RSQRT(DEF(sumsq)(col, 3) - x * x);
+        REAL sint = ((-z) * b) / (y * y + z * z);
+        REAL cost = RSQRT((REAL)1 - sint * sint);
+        M3(R2, 1, 1) = cost; M3(R2, 1, 2) = -sint;
+        M3(R2, 2, 1) = sint; M3(R2, 2, 2) = cost;
+    }
+    DEF(matmul)(R2, m, mout, 3);
+    DEF(matmul)(R2, R1, R, 3);
+}
+
+/* check_unit_cell: Box.jl:579-636; returns 1 when the plane distances exceed 2 rc */
+static int DEF(check_unit_cell)(const REAL *m, REAL rc, int dim) {
+    if (dim == 3) {
+        REAL a[3] = {M3(m, 0, 0), M3(m, 1, 0), M3(m, 2, 0)};
+        REAL b[3] = {M3(m, 0, 1), M3(m, 1, 1), M3(m, 2, 1)};
+        REAL c[3] = {M3(m, 0, 2), M3(m, 1, 2), M3(m, 2, 2)};
+        REAL bc[3], ab[3], ca[3];
+        DEF(cross)(b, c, bc);
+        REAL nbc = RSQRT(DEF(sumsq)(bc, 3));
+        for (int k = 0; k < 3; ++k) bc[k] = bc[k] / nbc;
+        REAL aproj = DEF(dot)(a, bc, 3);
+        DEF(cross)(a, b, ab);
+        REAL nab = RSQRT(DEF(sumsq)(ab, 3));
+        for (int k = 0; k < 3; ++k) ab[k] = ab[k] / nab;
+        REAL cproj = DEF(dot)(c, ab, 3);
+        DEF(cross)(c, a, ca);
+        REAL nca = RSQRT(DEF(sumsq)(ca, 3));
+        for (int k = 0; k < 3; ++k) ca[k] = ca[k] / nca;
+        REAL bproj = DEF(dot)(b, ca, 3);
+        REAL two_rc = 2 * rc;
+        if ((aproj <= two_rc) || (bproj <= two_rc) || (cproj <= two_rc)) return 0;
+        if (aproj != aproj || bproj != bproj || cproj != cproj) return 0;
+        return 1;
+    } else {
+        REAL a[2] = {M3(m, 0, 0), M3(m, 1, 0)};
+        REAL b[2] = {M3(m, 0, 1), M3(m, 1, 1)};
+        REAL na = RSQRT(DEF(sumsq)(a, 2));
+        REAL i[2] = {a[0] / na, a[1] / na};
+        REAL dbi = DEF(dot)(b, i, 2);
+        REAL bproj = RSQRT(DEF(sumsq)(b, 2) - dbi * dbi);
+        REAL nb = RSQRT(DEF(sumsq)(b, 2));
+        REAL j[2] = {b[0] / nb, b[1] / nb};
+        REAL daj = DEF(dot)(a, j, 2);
+        REAL aproj = RSQRT(DEF(sumsq)(a, 2) - daj * daj);
+        REAL two_rc = 2 * rc;
+        if ((aproj <= two_rc) || (bproj <= two_rc)) return 0;
+        if (aproj != aproj || bproj != bproj) return 0;
+        return 1;
+    }
+}
+
+/* cell_limits / cell_vertices: CellOperations.jl:431-479 */
+static void DEF(cell_limits)(const REAL *m, int dim, REAL *xmin, REAL *xmax) {
+    REAL v[8][3];
+    int nv;
+    REAL a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) { a[k] = M3(m, k, 0); b[k] = M3(m, k, 1); if (dim == 3) c[k] = M3(m, k, 2); }
+    if (dim == 2) {
+        nv = 4;
+        for (int k = 0; k < 2; ++k) { v[0][k] = 0; v[1][k] = a[k]; v[2][k] = a[k] + b[k]; v[3][k] = b[k]; }
+    } else {
+        nv = 8;
+        for (int k = 0; k < 3; ++k) {
+            v[0][k] = 0; v[1][k] = a[k]; v[2][k] = a[k] + b[k]; v[3][k] = b[k];
+            v[4][k] = a[k] + c[k]; v[5][k] = c[k]; v[6][k] = b[k] + c[k]; v[7][k] = (a[k] + b[k]) + c[k];
+        }
+    }
+    for (int k = 0; k < dim; ++k) { xmin[k] = v[0][k]; xmax[k] = v[0][k]; }
+    for (int iv = 1; iv < nv; ++iv)
+        for (int k = 0; k < dim; ++k) {
+            if (v[iv][k] < xmin[k]) xmin[k] = v[iv][k];
+            if (v[iv][k] > xmax[k]) xmax[k] = v[iv][k];
+        }
+}
+
+/*
+ * _construct_box + _compute_nc_and_cell_size: Box.jl:209-270.
+ * cellin: column-major dim x dim unit cell (columns = lattice vectors), already in REAL.
+ * Returns 0 on success, ORC_ERR_LCELL for lcell < 1, ORC_ERR_CELL when check_unit_cell fails.
+ */
+static int DEF(box_build)(int dim, const REAL *cellin, int celltype, REAL cutoff, int lcell,
+                          const REAL *origin, DEF(Box) *b) {
+    if (lcell < 1) return ORC_ERR_LCELL;
+    memset(b, 0, sizeof(*b));
+    b->dim = dim; b->celltype = celltype; b->lcell = lcell; b->cutoff = cutoff;
+    for (int i = 0; i < 9; ++i) b->input[i] = 0;
+    for (int j = 0; j < dim; ++j)
+        for (int i = 0; i < dim; ++i) M3(b->input, i, j) = cellin[i + dim * j];
+    if (celltype == ORC_TRICLINIC) {
+        if (dim == 3) DEF(align_cell3d)(b->input, b->aligned, b->rot);
+        else DEF(align_cell2d)(b->input, b->aligned, b->rot);
+    } else {
+        for (int i = 0; i < 9; ++i) b->aligned[i] = b->input[i];
+        DEF(identity)(b->rot);
+    }
+    if (!DEF(check_unit_cell)(b->aligned, cutoff, dim)) return ORC_ERR_CELL;
+    DEF(inverse)(b->rot, b->invrot, dim);
+    REAL xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
+    DEF(cell_limits)(b->aligned, dim, xmin, xmax);
+    REAL step = cutoff / (REAL)lcell;
+    for (int k = 0; k < dim; ++k) {
+        REAL q = (xmax[k] - xmin[k]) / step;
+        long _nc;
+        if (celltype == ORC_TRICLINIC) {
+            _nc = (long)RCEIL(q);
+            b->cs[k] = cutoff / (REAL)lcell;
+        } else {
+            _nc = (long)RFLOOR(q);
+            b->cs[k] = (xmax[k] - xmin[k]) / (REAL)_nc;
+        }
+        b->nc[k] = (int)(_nc + 2 * lcell + 1);
+        b->origin[k] = origin ? origin[k] : 0;
+        b->cbmin[k] = (xmin[k] + b->origin[k]) - (REAL)lcell * b->cs[k];
+        b->cbmax[k] = (xmax[k] + b->origin[k]) + (REAL)lcell * b->cs[k];
+    }
+    for (int k = dim; k < 3; ++k) b->nc[k] = 1;
+    b->cutoff_sqr = cutoff * cutoff;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* wrapping and cell assignment                                         */
+/* ------------------------------------------------------------------ */
+
+/* wrap_cell_fraction + wrap_to_first + rotation: CellOperations.jl:56-94, CellLists.jl:944-945 */
+static void DEF(wrap_rotate)(const DEF(Box) *b, const REAL *x, REAL *out) {
+    int n = b->dim;
+    REAL p[3];
+    DEF(solve)(b->input, x, p, n);
+    for (int k = 0; k < n; ++k) {
+        p[k] = p[k] - RFLOOR(p[k]);
+        if (p[k] == (REAL)1) p[k] = 0;
+    }
+    REAL w[3];
+    DEF(matvec)(b->input, p, w, n);
+    DEF(matvec)(b->rot, w, out, n);
+}
+
+/* particle_cell: Box.jl:499-508 (1-based cartesian index) */
+static void DEF(particle_cell)(const DEF(Box) *b, const REAL *x, long *c) {
+    for (int k = 0; k < b->dim; ++k) c[k] = (long)RFLOOR((x[k] - b->cbmin[k]) / b->cs[k]) + 1;
+}
+
+/* real_particle_border_case: CellLists.jl:956-967 */
+static void DEF(border_case)(const DEF(Box) *b, long *c) {
+    for (int k = 0; k < b->dim; ++k) {
+        if (c[k] == b->lcell) c[k] += 1;
+        if (c[k] == b->nc[k] - b->lcell + 1) c[k] -= 1;
+    }
+}
+
+/* in_computing_box: Box.jl:536-546 */
+static int DEF(in_box)(const DEF(Box) *b, const REAL *x) {
+    for (int k = 0; k < b->dim; ++k)
+        if (!(b->cbmin[k] <= x[k] && x[k] < b->cbmax[k])) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* cell list (serial insertion order of CellLists.jl:940-1057)          */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t index;   /* 0-based original index */
+    int real;
+    REAL x[3];
+} DEF(Entry);
+
+typedef struct {
+    int64_t n_real;
+    int64_t n_entries;
+    int64_t ncells;
+    DEF(Entry) *e;          /* entries grouped by cell, insertion order within a cell */
+    int64_t *cell_start;    /* ncells + 1 */
+    unsigned char *has_real;
+    int64_t *home;          /* cells with real particles, in order of discovery */
+    int64_t n_home;
+    int64_t n_cells_with_particles;
+} DEF(CellList);
+
+static void DEF(cl_free)(DEF(CellList) *cl) {
+    free(cl->e); free(cl->cell_start); free(cl->has_real); free(cl->home);
+    memset(cl, 0, sizeof(*cl));
+}
+
+static int64_t DEF(lin)(const DEF(Box) *b, const long *c1) {
+    int64_t li = c1[0] - 1;
+    if (b->dim >= 2) li += (int64_t)b->nc[0] * (c1[1] - 1);
+    if (b->dim >= 3) li += (int64_t)b->nc[0] * b->nc[1] * (c1[2] - 1);
+    return li;
+}
+
+/*
+ * UpdateCellList! (serial path, CellLists.jl:644-662 + add_particles! :940-950,
+ * add_particle_to_celllist! :983-1057, replicate_particle! Box.jl:556-566) and the
+ * non-periodic add_particles! (NonPeriodicCells.jl:63-70: no wrap, no images).
+ * Returns 0, ORC_ERR_NAN for NaN/Inf coordinates, ORC_ERR_RANGE for a cell index
+ * outside the grid (where the reference would raise a BoundsError), ORC_ERR_NOMEM.
+ */
+static int DEF(cl_build)(const DEF(Box) *b, const REAL *x, int64_t n, DEF(CellList) *cl) {
+    memset(cl, 0, sizeof(*cl));
+    int D = b->dim;
+    int64_t ncells = (int64_t)b->nc[0] * b->nc[1] * b->nc[2];
+    cl->ncells = ncells;
+    cl->n_real = n;
+    for (int64_t i = 0; i < n * D; ++i)
+        if (!isfinite((double)x[i])) return ORC_ERR_NAN;
+    int64_t cap = n * 4 + 16, cnt = 0;
+    DEF(Entry) *tmp = (DEF(Entry) *)malloc(sizeof(DEF(Entry)) * cap);
+    int64_t *tcell = (int64_t *)malloc(sizeof(int64_t) * cap);
+    cl->has_real = (unsigned char *)calloc(ncells, 1);
+    cl->home = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+    cl->cell_start = (int64_t *)calloc(ncells + 1, sizeof(int64_t));
+    if (!tmp || !tcell || !cl->has_real || !cl->home || !cl->cell_start) return ORC_ERR_NOMEM;
+    int rc = 0;
+    int nimg = (D == 3) ? 27 : 9;
+    for (int64_t ip = 0; ip < n && rc == 0; ++ip) {
+        REAL p[3] = {0, 0, 0};
+        if (b->celltype == ORC_NONPERIODIC) {
+            for (int k = 0; k < D; ++k) p[k] = x[ip * D + k];
+        } else {
+            REAL xin[3] = {0, 0, 0};
+            for (int k = 0; k < D; ++k) xin[k] = x[ip * D + k];
+            DEF(wrap_rotate)(b, xin, p);
+        }
+        for (int im = -1; im < nimg; ++im) {
+            /* im == -1: the real particle; then images in Iterators.product(-1:1,...) order */
+            REAL q[3] = {p[0], p[1], p[2]};
+            int real = (im < 0);
+            if (!real) {
+                if (b->celltype == ORC_NONPERIODIC) break;
+                int k1 = im % 3 - 1, k2 = (im / 3) % 3 - 1, k3 = (D == 3) ? (im / 9) - 1 : 0;
+                if (k1 == 0 && k2 == 0 && k3 == 0) continue;
+                REAL kv[3] = {(REAL)k1, (REAL)k2, (REAL)k3};
+                REAL t[3];
+                DEF(matvec)(b->aligned, kv, t, D);
+                for (int k = 0; k < D; ++k) q[k] = p[k] + t[k];
+                if (!DEF(in_box)(b, q)) continue;
+            }
+            long c[3] = {1, 1, 1};
+            DEF(particle_cell)(b, q, c);
+            if (real) DEF(border_case)(b, c);
+            for (int k = 0; k < D; ++k)
+                if (c[k] < 1 || c[k] > b->nc[k]) rc = ORC_ERR_RANGE;
+            if (rc) break;
+            int64_t li = DEF(lin)(b, c);
+            if (cnt == cap) {
+                cap *= 2;
+                tmp = (DEF(Entry) *)realloc(tmp, sizeof(DEF(Entry)) * cap);
+                tcell = (int64_t *)realloc(tcell, sizeof(int64_t) * cap);
+                if (!tmp || !tcell) return ORC_ERR_NOMEM;
+            }
+            tmp[cnt].index = ip;
+            tmp[cnt].real = real;
+            for (int k = 0; k < 3; ++k) tmp[cnt].x[k] = q[k];
+            tcell[cnt] = li;
+            cnt++;
+            cl->cell_start[li + 1]++;
+            if (real && !cl->has_real[li]) {
+                cl->has_real[li] = 1;
+                cl->home[cl->n_home++] = li;
+            }
+        }
+    }
+    if (rc) { free(tmp); free(tcell); return rc; }
+    for (int64_t c = 0; c < ncells; ++c) {
+        if (cl->cell_start[c + 1] > 0) cl->n_cells_with_particles++;
+        cl->cell_start[c + 1] += cl->cell_start[c];
+    }
+    int64_t *cursor = (int64_t *)malloc(sizeof(int64_t) * (ncells + 1));
+    cl->e = (DEF(Entry) *)malloc(sizeof(DEF(Entry)) * (cnt + 1));
+    if (!cursor || !cl->e) return ORC_ERR_NOMEM;
+    memcpy(cursor, cl->cell_start, sizeof(int64_t) * (ncells + 1));
+    for (int64_t k = 0; k < cnt; ++k) cl->e[cursor[tcell[k]]++] = tmp[k];
+    cl->n_entries = cnt;
+    free(cursor); free(tmp); free(tcell);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* the sweep                                                            */
+/* ------------------------------------------------------------------ */
+
+typedef void (*DEF(pair_cb))(void *ctx, int64_t i, int64_t j, const REAL *xi, const REAL *xj, REAL d2);
+
+/* cell_center: Box.jl:517-525 (1-based c) */
+static void DEF(cell_center)(const DEF(Box) *b, const long *c, REAL *out) {
+    for (int k = 0; k < b->dim; ++k) out[k] = (b->cbmin[k] + b->cs[k] * (REAL)c[k]) - b->cs[k] / 2;
+}
+
+static void DEF(cartesian)(const DEF(Box) *b, int64_t li, long *c) {
+    c[0] = li % b->nc[0] + 1;
+    c[1] = (li / b->nc[0]) % b->nc[1] + 1;
+    c[2] = li / ((int64_t)b->nc[0] * b->nc[1]) + 1;
+}
+
+static REAL DEF(d2of)(const REAL *a, const REAL *c, int D) {
+    REAL d[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) d[k] = a[k] - c[k];
+    return DEF(sumsq)(d, D);
+}
+
+/*
+ * One home cell: _current_cell_interactions! (self.jl:143-184, NonPeriodicCells.jl:281-297)
+ * then inner_loop! over the stencil (self.jl:116-135; Box.jl:436-474) with
+ * _vicinal_cell_interactions! / project_particles! / _vinicial_cells!
+ * (vicinal_cells.jl:4-75, auxiliary_functions.jl:66-111, NonPeriodicCells.jl:330-352).
+ *
+ * faithful = 1 applies the projection filters exactly as the reference does;
+ * faithful = 0 evaluates every candidate of the same cell pairs (pure d2 <= rc^2).
+ * The two modes produce the same pair set except for a within-cutoff pair whose
+ * rounded projection difference exceeds rc; the test suite checks that they agree.
+ */
+static void DEF(sweep_cell)(const DEF(Box) *b, const DEF(CellList) *cl, int64_t li, int faithful,
+                            DEF(pair_cb) cb, void *ctx, REAL *pbuf_x, int64_t *pbuf_i) {
+    int D = b->dim;
+    int L = b->lcell;
+    int tric = (b->celltype == ORC_TRICLINIC);
+    int nonper = (b->celltype == ORC_NONPERIODIC);
+    REAL rc = b->cutoff, rc2 = b->cutoff_sqr;
+    int64_t s0 = cl->cell_start[li], s1 = cl->cell_start[li + 1];
+    /* self cell */
+    for (int64_t a = s0; a < s1; ++a) {
+        const DEF(Entry) *p = &cl->e[a];
+        if (tric) {
+            if (!p->real) continue;
+            for (int64_t c = s0; c < s1; ++c) {
+                const DEF(Entry) *q = &cl->e[c];
+                if (p->index >= q->index) continue;
+                REAL d2 = DEF(d2of)(p->x, q->x, D);
+                if (d2 <= rc2) cb(ctx, p->index, q->index, p->x, q->x, d2);
+            }
+        } else {
+            for (int64_t c = a + 1; c < s1; ++c) {
+                const DEF(Entry) *q = &cl->e[c];
+                if (!nonper && !(p->real | q->real)) continue;
+                REAL d2 = DEF(d2of)(p->x, q->x, D);
+                if (d2 <= rc2) cb(ctx, p->index, q->index, p->x, q->x, d2);
+            }
+        }
+    }
+    long ci[3];
+    DEF(cartesian)(b, li, ci);
+    REAL cen_i[3] = {0, 0, 0};
+    DEF(cell_center)(b, ci, cen_i);
+    /* stencil offsets, generated in the reference's iteration order */
+    int w = 2 * L + 1;
+    int ntot = (D == 3) ? w * w * w : w * w;
+    for (int t = 0; t < ntot; ++t) {
+        long off[3] = {0, 0, 0};
+        /* CartesianIndices, first index fastest */
+        off[0] = t % w - L;
+        off[1] = (t / w) % w - L;
+        off[2] = (D == 3) ? (t / (w * w)) - L : 0;
+        if (off[0] == 0 && off[1] == 0 && off[2] == 0) continue;
+        if (!tric) {
+            /* neighbor_cells_forward: (1:L, -L:L, -L:L) U (0, 1:L, -L:L) U (0, 0, 1:L) */
+            int fwd;
+            if (D == 3) fwd = (off[0] >= 1) || (off[0] == 0 && off[1] >= 1) || (off[0] == 0 && off[1] == 0 && off[2] >= 1);
+            else fwd = (off[0] >= 1) || (off[0] == 0 && off[1] >= 1);
+            if (!fwd) continue;
+        }
+        long cj[3] = {ci[0] + off[0], ci[1] + off[1], ci[2] + off[2]};
+        int64_t lj = DEF(lin)(b, cj);
+        int64_t t0 = cl->cell_start[lj], t1 = cl->cell_start[lj + 1];
+        if (t1 == t0) continue;
+        REAL u[3] = {0, 0, 0}, cen_j[3] = {0, 0, 0}, dc[3] = {0, 0, 0};
+        int64_t np = 0;
+        double margin = 0;
+        REAL dcn = 0;
+        if (faithful) {
+            DEF(cell_center)(b, cj, cen_j);
+            for (int k = 0; k < D; ++k) dc[k] = cen_j[k] - cen_i[k];
+            dcn = RSQRT(DEF(sumsq)(dc, D));
+            for (int k = 0; k < D; ++k) u[k] = dc[k] / dcn;
+            if (L == 1) margin = (double)(rc + dcn / 2);
+            else margin = (double)rc * (1.0 + sqrt((double)D) / 2.0);
+            for (int64_t c = t0; c < t1; ++c) {
+                REAL d[3] = {0, 0, 0};
+                for (int k = 0; k < D; ++k) d[k] = cl->e[c].x[k] - cen_i[k];
+                REAL xp = DEF(dot)(d, u, D);
+                if ((double)RFABS(xp) <= margin) { pbuf_x[np] = xp; pbuf_i[np] = c; np++; }
+            }
+        } else {
+            for (int64_t c = t0; c < t1; ++c) { pbuf_i[np] = c; np++; }
+        }
+        for (int64_t a = s0; a < s1; ++a) {
+            const DEF(Entry) *p = &cl->e[a];
+            if (tric && !p->real) continue;
+            REAL xpi = 0;
+            if (faithful) {
+                REAL d[3] = {0, 0, 0};
+                for (int k = 0; k < D; ++k) d[k] = p->x[k] - cen_i[k];
+                xpi = DEF(dot)(d, u, D);
+            }
+            for (int64_t m = 0; m < np; ++m) {
+                if (faithful && !(RFABS(pbuf_x[m] - xpi) <= rc)) continue;
+                const DEF(Entry) *q = &cl->e[pbuf_i[m]];
+                if (tric) {
+                    if (p->index >= q->index) continue;
+                } else if (!nonper) {
+                    if (!(p->real | q->real)) continue;
+                }
+                REAL d2 = DEF(d2of)(p->x, q->x, D);
+                if (d2 <= rc2) cb(ctx, p->index, q->index, p->x, q->x, d2);
+            }
+        }
+    }
+}
+
+/* _pairwise_serial! (self.jl:50-63): every home cell, or every stride-th one starting at offset */
+static int DEF(sweep)(const DEF(Box) *b, const DEF(CellList) *cl, int faithful, DEF(pair_cb) cb, void *ctx,
+                      int64_t stride, int64_t offset) {
+    int64_t maxn = 0;
+    for (int64_t c = 0; c < cl->ncells; ++c) {
+        int64_t k = cl->cell_start[c + 1] - cl->cell_start[c];
+        if (k > maxn) maxn = k;
+    }
+    REAL *px = (REAL *)malloc(sizeof(REAL) * (maxn + 1));
+    int64_t *pi = (int64_t *)malloc(sizeof(int64_t) * (maxn + 1));
+    if (!px || !pi) { free(px); free(pi); return ORC_ERR_NOMEM; }
+    if (stride < 1) stride = 1;
+    for (int64_t h = offset; h < cl->n_home; h += stride) DEF(sweep_cell)(b, cl, cl->home[h], faithful, cb, ctx, px, pi);
+    free(px); free(pi);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* brute force: Testing.map_naive! (test/modules/Testing.jl:76-89) with
+ * wrap_relative_to (CellOperations.jl:102-127)                         */
+/* ------------------------------------------------------------------ */
+
+static void DEF(wrap_fraction)(const REAL *m, const REAL *x, REAL *p, int n) {
+    DEF(solve)(m, x, p, n);
+    for (int k = 0; k < n; ++k) {
+        p[k] = p[k] - RFLOOR(p[k]);
+        if (p[k] == (REAL)1) p[k] = 0;
+    }
+}
+
+/* Julia mod(x, 1) for floats: r = rem(x, 1); r != 0 and negative -> r + 1 */
+static REAL DEF(jmod1)(REAL x) {
+    REAL r = RFMOD(x, (REAL)1);
+    if (r == 0) return (REAL)0;
+    if (r < 0) return r + (REAL)1;
+    return r;
+}
+
+static int DEF(naive)(const DEF(Box) *b, const REAL *x, int64_t n, DEF(pair_cb) cb, void *ctx) {
+    int D = b->dim;
+    REAL *f = (REAL *)malloc(sizeof(REAL) * 3 * (n + 1));
+    if (!f) return ORC_ERR_NOMEM;
+    for (int64_t i = 0; i < n; ++i) {
+        REAL xin[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) xin[k] = x[i * D + k];
+        DEF(wrap_fraction)(b->input, xin, &f[3 * i], D);
+    }
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        const REAL *xi = &x[i * D];
+        const REAL *fi = &f[3 * i];
+        for (int64_t j = i + 1; j < n; ++j) {
+            const REAL *fj = &f[3 * j];
+            REAL xw[3], dfr[3], xj[3], tmp[3];
+            for (int k = 0; k < D; ++k) {
+                REAL v = DEF(jmod1)(fj[k] - fi[k]);
+                if (v >= (REAL)0.5) v = v - 1;
+                xw[k] = v + fi[k];
+                dfr[k] = xw[k] - fi[k];
+            }
+            DEF(matvec)(b->input, dfr, tmp, D);
+            for (int k = 0; k < D; ++k) xj[k] = tmp[k] + xi[k];
+            REAL d2 = DEF(d2of)(xi, xj, D);
+            if (d2 <= b->cutoff_sqr) cb(ctx, i, j, xi, xj, d2);
+        }
+    }
+    free(f);
+    return 0;
+}
+
+#undef DEF
