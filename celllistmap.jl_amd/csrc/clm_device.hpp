@@ -55,19 +55,19 @@ __device__ inline float dev_floor<float>(float v) { return floorf(v); }
 template <>
 __device__ inline double dev_floor<double>(double v) { return floor(v); }
 
+// Correctly rounded square root and division.  The translation unit is compiled with
+// -fhip-fp32-correctly-rounded-divide-sqrt, under which llvm.sqrt.f32 and fdiv are
+// lowered to IEEE round-to-nearest sequences (HIP's __fsqrt_rn maps to the native,
+// 1-ulp instruction unless OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used).
 template <typename T>
 __device__ inline T dev_sqrt_rn(T v);
 template <>
-__device__ inline float dev_sqrt_rn<float>(float v) { return __fsqrt_rn(v); }
+__device__ inline float dev_sqrt_rn<float>(float v) { return __builtin_sqrtf(v); }
 template <>
-__device__ inline double dev_sqrt_rn<double>(double v) { return __dsqrt_rn(v); }
+__device__ inline double dev_sqrt_rn<double>(double v) { return __builtin_sqrt(v); }
 
 template <typename T>
-__device__ inline T dev_div_rn(T a, T b);
-template <>
-__device__ inline float dev_div_rn<float>(float a, float b) { return __fdiv_rn(a, b); }
-template <>
-__device__ inline double dev_div_rn<double>(double a, double b) { return __ddiv_rn(a, b); }
+__device__ inline T dev_div_rn(T a, T b) { return a / b; }
 
 template <typename T>
 __device__ inline bool dev_finite(T v) { return isfinite(v); }
