@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--lcell", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ef-reps", type=int, default=5)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is a "
@@ -95,6 +97,12 @@ def main():
     clm = graft.load_package()
     dev = torch.device("cuda:0")
     rc = 12.0
+    if args.copy_calibration:
+        big = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(big)
+        dst.copy_(big)
+        torch.cuda.synchronize()
+        del big, dst
     M, x0 = make_workload(args.n)
     x = torch.from_numpy(x0).to(dev)
     nsteps = args.warmup + args.steps

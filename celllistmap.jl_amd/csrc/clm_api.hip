@@ -70,6 +70,7 @@ struct clm_ctx {
 
     // operator buffers
     DBuf nl_i, nl_j, nl_d, nl_count, outbuf, acc, hist_c, hist_v, md_d2, md_a, md_b;
+    DBuf sp_i, sp_j, sp_d, ch_used, ch_holes, ch_movers, ch_hstart, ch_mstart;
     uint32_t *h_flags = nullptr;
 
     // measurement
@@ -346,8 +347,8 @@ clm_status update_cell_list_t(clm_ctx *c) {
     if (*c->h_flags & FLAG_RANGE)
         return fail(c, CLM_ERR_RANGE, "a particle maps outside the computing grid (the reference raises a BoundsError)");
     c->n_entries = total;
-    HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * std::max<int64_t>(total, 1)));
-    c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>));
+    HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
+    c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
     if ((s = fill_int(c, c->cur_real.p, nc1, 0))) return s;
     if ((s = fill_int(c, c->cur_ghost.p, nc1, 0))) return s;
     if (n > 0) {
@@ -418,6 +419,14 @@ SweepArgs<T> sweep_args(clm_ctx *c, int mode) {
     return sa;
 }
 
+// Persistent grid: as many one-wave workgroups as can be resident on every CU.
+template <typename KernelT>
+int occupancy_grid(clm_ctx *c, KernelT kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess || per_cu <= 0) per_cu = 16;
+    per_cu = std::min(per_cu, 32);
+    return std::max(64, c->num_cus * per_cu);
+}
 int sweep_grid(clm_ctx *c) { return std::max(64, c->num_cus * 16); }
 
 template <typename T, int MODE, typename Op>
@@ -426,6 +435,7 @@ clm_status run_sweep(clm_ctx *c, const char *name, const Op &op, int grid) {
     clm_status s;
     if ((s = fill_int(c, c->counters.p, 16, 0))) return s;
     SweepArgs<T> sa = sweep_args<T>(c, MODE);
+    if (grid <= 0) grid = c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE, 2, Op>);
     if (c->dim == 3)
         LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, sa.E, sa.cstart, sa, op));
     else
@@ -455,21 +465,54 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         dj = (int32_t *)c->nl_j.p;
         dd = (T *)c->nl_d.p;
     }
-    HIPCHK(c, ensure(c->nl_count, sizeof(unsigned long long)));
-    if ((s = fill_int(c, c->nl_count.p, 2, 0))) return s;
+    // persistent grid and a bound on the slots left unused in partially filled chunks
+    int grid;
+    if (c->dim == 3)
+        grid = ma == MODE_A_TRIC ? occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 3, OpNeighborList<T>>)
+                                 : occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 3, OpNeighborList<T>>);
+    else
+        grid = ma == MODE_A_TRIC ? occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 2, OpNeighborList<T>>)
+                                 : occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 2, OpNeighborList<T>>);
+    const int64_t ub_tasks = c->n_entries / 64 + c->nrows + 1;
+    const int64_t active = std::min<int64_t>(grid, ub_tasks);
+    const int64_t scap = 2 * (int64_t)NL_CHUNK * active + NL_CHUNK;
+    const int64_t nchunks = (cap + scap + NL_CHUNK - 1) / NL_CHUNK + 1;
+    HIPCHK(c, ensure(c->sp_i, sizeof(int32_t) * scap));
+    HIPCHK(c, ensure(c->sp_j, sizeof(int32_t) * scap));
+    HIPCHK(c, ensure(c->sp_d, sizeof(T) * scap));
+    for (DBuf *b : {&c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart})
+        HIPCHK(c, ensure(*b, sizeof(int) * (nchunks + 1)));
+    HIPCHK(c, ensure(c->nl_count, sizeof(unsigned long long) * 2));
+    if ((s = fill_int(c, c->nl_count.p, 4, 0))) return s;
+    if ((s = fill_int(c, c->ch_used.p, nchunks + 1, 0))) return s;
+    NLOut<T> out;
+    out.oi = di; out.oj = dj; out.od = dd;
+    out.si = (int32_t *)c->sp_i.p; out.sj = (int32_t *)c->sp_j.p; out.sd = (T *)c->sp_d.p;
+    out.cap = cap;
+    out.scap = scap;
     OpNeighborList<T> op;
-    op.oi = di;
-    op.oj = dj;
-    op.od = dd;
-    op.count = (unsigned long long *)c->nl_count.p;
-    op.cap = cap;
-    if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, sweep_grid(c)))) return s;
+    std::memset((void *)&op, 0, sizeof(op));
+    op.out = out;
+    op.chunk_counter = (unsigned long long *)c->nl_count.p;
+    op.total = (unsigned long long *)c->nl_count.p + 1;
+    op.chunk_used = (int *)c->ch_used.p;
+    op.max_chunks = nchunks;
+    if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, grid))) return s;
+    const unsigned long long *tot = (const unsigned long long *)c->nl_count.p + 1;
+    LAUNCH(c, "k_nl_holes", hipLaunchKernelGGL(k_nl_holes, dim3(blocks(nchunks + 1, 256)), dim3(256), 0, c->stream,
+                                               (long long)nchunks, (const int *)c->ch_used.p, tot, (long long)cap,
+                                               (int *)c->ch_holes.p, (int *)c->ch_movers.p));
+    if ((s = scan_int(c, (const int *)c->ch_holes.p, (int *)c->ch_hstart.p, (int)nchunks + 1))) return s;
+    if ((s = scan_int(c, (const int *)c->ch_movers.p, (int *)c->ch_mstart.p, (int)nchunks + 1))) return s;
+    LAUNCH(c, "k_nl_fill", hipLaunchKernelGGL((k_nl_fill<T>), dim3(std::max(1, std::min(2048, c->num_cus * 8))), dim3(256), 0,
+                                              c->stream, (long long)nchunks, (const int *)c->ch_used.p,
+                                              (const int *)c->ch_hstart.p, (const int *)c->ch_mstart.p, tot, out));
     unsigned long long cnt = 0;
-    HIPCHK(c, hipMemcpyAsync(&cnt, c->nl_count.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&cnt, tot, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     if ((s = sync(c))) return s;
     *npairs = (int64_t)cnt;
-    if (!on_device && cnt > 0) {
-        const int64_t m = std::min<int64_t>((int64_t)cnt, cap);
+    if (!on_device && cnt > 0 && (int64_t)cnt <= cap) {
+        const int64_t m = (int64_t)cnt;
         HIPCHK(c, hipMemcpyAsync(oi, di, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(oj, dj, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(od, dd, sizeof(T) * m, hipMemcpyDeviceToHost, c->stream));
@@ -527,7 +570,7 @@ clm_status softcore_t(clm_ctx *c, const clm_softcore *p, double *energy, void *f
     op.tric = c->celltype == CLM_TRICLINIC;
     op.dim = D;
     const char *name = ADJ ? "k_sweep_energy_adjoint" : (FORCES ? "k_sweep_energy_forces" : "k_sweep_energy");
-    if ((s = run_sweep<T, MODE_B>(c, name, op, sweep_grid(c)))) return s;
+    if ((s = run_sweep<T, MODE_B>(c, name, op, 0))) return s;
     double acc[24];
     HIPCHK(c, hipMemcpyAsync(acc, c->acc.p, sizeof(double) * 24, hipMemcpyDeviceToHost, c->stream));
     if ((FORCES || ADJ) && fout && !on_device && c->n > 0)
@@ -568,7 +611,7 @@ clm_status histogram_t(clm_ctx *c, int nbins, double width, int64_t *counts, dou
     op.dim = c->dim;
     op.counts = (unsigned long long *)c->hist_c.p;
     op.vsum = (double *)c->hist_v.p;
-    if ((s = run_mode_a<T>(c, "k_sweep_histogram", op, sweep_grid(c)))) return s;
+    if ((s = run_mode_a<T>(c, "k_sweep_histogram", op, 0))) return s;
     std::vector<unsigned long long> hc(nbins);
     HIPCHK(c, hipMemcpyAsync(hc.data(), c->hist_c.p, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost, c->stream));
     if (vsum) HIPCHK(c, hipMemcpyAsync(vsum, c->hist_v.p, sizeof(double) * nbins, hipMemcpyDeviceToHost, c->stream));
@@ -653,7 +696,8 @@ clm_status clm_destroy(clm_ctx *c) {
                     &c->cur_real, &c->cur_ghost, &c->E, &c->flags, &c->hc, &c->ntask, &c->hstart[0],
                     &c->hstart[1], &c->hstart[2], &c->tstart[0], &c->tstart[1], &c->tstart[2], &c->counters,
                     &c->scan_tmp, &c->nl_i, &c->nl_j, &c->nl_d, &c->nl_count, &c->outbuf, &c->acc,
-                    &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b};
+                    &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b, &c->sp_i, &c->sp_j, &c->sp_d,
+                    &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart};
     for (DBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

@@ -187,13 +187,22 @@ __device__ inline int fetch_task(int *counters, int ntasks, int &home_xcd, int &
     return -1;
 }
 
+// Look-ahead batch of neighbour entries held in scalar registers.
+template <typename T>
+struct SweepBatch {
+    static constexpr int value = sizeof(T) == 4 ? 8 : 4;
+};
+constexpr int SWEEP_PAD = 16;  // entries of padding after E (>= 2 * batch)
+
 template <typename T, int MODE, int D, typename Op>
 __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, const int *__restrict__ cs,
-                                              const SweepArgs<T> sa, Op op) {
+                                              const SweepArgs<T> sa, const Op op) {
     __shared__ typename Op::Shared sh;
     __shared__ int s_task;
     const int lane = threadIdx.x;
-    op.wave_init(sh);
+    constexpr int B = SweepBatch<T>::value;
+    typename Op::Wave ws;
+    op.wave_init(sh, ws);
     const int ncx = sa.nc[0], ncy = sa.nc[1];
     const int L = sa.lcell;
     const int ntasks = sa.tstart[sa.nrows];
@@ -223,7 +232,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
         const uint32_t ip = wp & IDX_MASK;
         const bool preal = !(wp & GHOST_BIT);
         typename Op::Lane ls;
-        op.lane_init(ls, p, valid);
+        op.lane_init(ws, ls, p, valid);
         const int dzr = (D == 3) ? L : 0;
         for (int dz = -dzr; dz <= dzr; ++dz) {
             for (int dy = -L; dy <= L; ++dy) {
@@ -238,72 +247,158 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                 int J0 = uniform_int(cs[base + cxmin + (MODE == MODE_A_ORTHO && selfrow ? 0 : xlo)]);
                 int J1 = uniform_int(__shfl(cs[base + cx + L + 1], nvalid - 1, 64));
                 (void)cxmax;
-                for (int j = J0; j < J1; ++j) {
-                    const Ent<T> q = E[j];
-                    const uint32_t wq = ent_w(q);
-                    const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
-                    T d2 = dx * dx + dy2 * dy2;
-                    if (D == 3) d2 = d2 + dz2 * dz2;
-                    bool ok = d2 <= sa.rc2;
-                    if (MODE == MODE_A_ORTHO) {
-                        ok = ok && (j >= lo) && (j < hi) && (preal || !(wq & GHOST_BIT));
-                    } else if (MODE == MODE_A_TRIC) {
-                        ok = ok && (j >= lo) && (j < hi) && (ip < (wq & IDX_MASK));
-                    } else {
-                        ok = ok && valid && (ip != (wq & IDX_MASK));
+                // Neighbour entries are wave-uniform: they are read with scalar loads in
+                // batches of B, the next batch issued before the current one is used (E is
+                // padded by 2*B entries, so the look-ahead never leaves the allocation).
+                Ent<T> cur[B], nxt[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u) cur[u] = E[J0 + u];
+                for (int j0 = J0; j0 < J1; j0 += B) {
+#pragma unroll
+                    for (int u = 0; u < B; ++u) nxt[u] = E[j0 + B + u];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        const int j = j0 + u;
+                        if (j < J1) {
+                            const Ent<T> &q = cur[u];
+                            const uint32_t wq = ent_w(q);
+                            const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
+                            T d2 = dx * dx + dy2 * dy2;
+                            if (D == 3) d2 = d2 + dz2 * dz2;
+                            bool ok = d2 <= sa.rc2;
+                            if (MODE == MODE_A_ORTHO) {
+                                ok = ok && (j >= lo) && (j < hi) && (preal || !(wq & GHOST_BIT));
+                            } else if (MODE == MODE_A_TRIC) {
+                                ok = ok && (j >= lo) && (j < hi) && (ip < (wq & IDX_MASK));
+                            } else {
+                                ok = ok && valid && (ip != (wq & IDX_MASK));
+                            }
+                            op.candidate(sh, ws, ls, ok, p, q, d2);
+                        }
                     }
-                    op.candidate(sh, ls, ok, p, q, d2);
+#pragma unroll
+                    for (int u = 0; u < B; ++u) cur[u] = nxt[u];
                 }
             }
         }
-        op.lane_end(sh, ls, p, valid);
+        op.lane_end(sh, ws, ls, p, valid);
     }
-    op.wave_end(sh);
+    op.wave_end(sh, ws);
 }
 
 // ------------------------------------------------------------------------------
 // operators
 // ------------------------------------------------------------------------------
 
-// Neighbour list: hits are staged in LDS and flushed in coalesced blocks; one global
-// atomic per flush reserves the output range (push_pair!, internals/neighborlist.jl:67-76).
+// Neighbour list (push_pair!, internals/neighborlist.jl:67-76).  Hits are staged in
+// LDS and written out in coalesced blocks.  Output space is handed out in chunks of CH
+// slots: a wave owns one chunk at a time and requests the next one with a single
+// device-scope atomic issued well before it is needed, so the atomic's latency is hidden
+// and there are ~P/CH atomics instead of one per flush.  Slots at or beyond the caller's
+// capacity go to a library-owned spill area.  The partially used last chunk of every wave
+// leaves holes; k_nl_holes / k_nl_fill move entries from beyond the pair count into them,
+// so the final list is dense (its order is unspecified, as in the reference).
+constexpr int NL_CHUNK = 4096;
+
+template <typename T>
+struct NLOut {
+    int32_t *oi, *oj;
+    T *od;
+    int32_t *si, *sj;
+    T *sd;
+    long long cap, scap;
+    __device__ void put(long long s, int i, int j, T d) const {
+        const bool u = s < cap;
+        const long long q = u ? s : s - cap;
+        int32_t *pi = u ? oi : si;
+        int32_t *pj = u ? oj : sj;
+        T *pd = u ? od : sd;
+        if (u || q < scap) {
+            pi[q] = i; pj[q] = j; pd[q] = d;
+        }
+    }
+    __device__ void get(long long s, int &i, int &j, T &d) const {
+        const bool u = s < cap;
+        const long long q = u ? s : s - cap;
+        const int32_t *pi = u ? oi : si;
+        const int32_t *pj = u ? oj : sj;
+        const T *pd = u ? od : sd;
+        i = pi[q]; j = pj[q]; d = pd[q];
+    }
+};
+
+__device__ inline unsigned long long uniform_u64(unsigned long long v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 template <typename T>
 struct OpNeighborList {
     static constexpr int S = 512;
+    static constexpr int CH = NL_CHUNK;
+    static constexpr int PREFETCH = 1536;  // request the next chunk when fewer slots remain
     struct Shared {
         int i[S];
         int j[S];
         T d2[S];
-        unsigned long long base;
     };
     struct Lane {
         int n;  // staged hits (same value in every lane)
     };
-    int32_t *oi;
-    int32_t *oj;
-    T *od;
-    unsigned long long *count;
-    int64_t cap;
-    int staged;  // staged hits carried from one task to the next (kept in a register)
+    struct Wave {  // wave-uniform state carried across tasks
+        int staged;
+        int cur_used;
+        long long cur;
+        unsigned long long nxt_raw;  // lane 0 holds the pending atomic result
+        unsigned long long written;
+        bool have_nxt;
+    };
+    NLOut<T> out;
+    unsigned long long *chunk_counter;
+    unsigned long long *total;
+    int *chunk_used;
+    long long max_chunks;
 
-    __device__ void wave_init(Shared &) { staged = 0; }
-    __device__ void lane_init(Lane &l, const Ent<T> &, bool) { l.n = staged; }
-    __device__ void flush(Shared &sh, int n) {
+    __device__ void request(Wave &w) const {
+        w.nxt_raw = threadIdx.x == 0 ? atomicAdd(chunk_counter, 1ull) : 0ull;
+        w.have_nxt = true;
+    }
+    __device__ void wave_init(Shared &, Wave &w) const {
+        w.staged = 0;
+        w.cur = -1;
+        w.cur_used = CH;
+        w.written = 0;
+        w.nxt_raw = 0;
+        w.have_nxt = false;  // the first chunk is requested at the first flush
+    }
+    __device__ void lane_init(Wave &w, Lane &l, const Ent<T> &, bool) const { l.n = w.staged; }
+    __device__ void retire(const Wave &w, int used) const {
+        if (w.cur >= 0 && w.cur < max_chunks && threadIdx.x == 0) chunk_used[w.cur] = used;
+    }
+    __device__ void flush(Shared &sh, Wave &w, int n) const {
         __syncthreads();
-        if (threadIdx.x == 0) sh.base = atomicAdd(count, (unsigned long long)n);
-        __syncthreads();
-        const long long base = (long long)sh.base;
-        for (int k = threadIdx.x; k < n; k += 64) {
-            long long q = base + k;
-            if (q < cap) {
-                oi[q] = sh.i[k];
-                oj[q] = sh.j[k];
-                od[q] = dev_sqrt_rn(sh.d2[k]);
+        int k = 0;
+        while (k < n) {
+            if (w.cur_used == CH) {
+                retire(w, CH);
+                if (!w.have_nxt) request(w);
+                w.cur = (long long)uniform_u64(w.nxt_raw);
+                w.cur_used = 0;
+                w.have_nxt = false;
             }
+            const int m = min(n - k, CH - w.cur_used);
+            const long long base = w.cur * CH + w.cur_used;
+            for (int t = threadIdx.x; t < m; t += 64)
+                out.put(base + t, sh.i[k + t], sh.j[k + t], dev_sqrt_rn(sh.d2[k + t]));
+            w.cur_used += m;
+            k += m;
         }
+        w.written += (unsigned long long)n;
+        if (!w.have_nxt && w.cur_used > CH - PREFETCH) request(w);
         __syncthreads();
     }
-    __device__ void candidate(Shared &sh, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+    __device__ void candidate(Shared &sh, Wave &w, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         const uint64_t m = ballot64(ok);
         if (m == 0) return;
         if (ok) {
@@ -314,15 +409,68 @@ struct OpNeighborList {
         }
         l.n += __popcll(m);
         if (l.n > S - 64) {
-            flush(sh, l.n);
+            flush(sh, w, l.n);
             l.n = 0;
         }
     }
-    __device__ void lane_end(Shared &, Lane &l, const Ent<T> &, bool) { staged = l.n; }
-    __device__ void wave_end(Shared &sh) {
-        if (staged > 0) flush(sh, staged);
+    __device__ void lane_end(Shared &, Wave &w, Lane &l, const Ent<T> &, bool) const { w.staged = l.n; }
+    __device__ void wave_end(Shared &sh, Wave &w) const {
+        if (w.staged > 0) flush(sh, w, w.staged);
+        if (w.cur >= 0) retire(w, w.cur_used);
+        if (w.have_nxt) {
+            const long long nx = (long long)uniform_u64(w.nxt_raw);
+            if (nx < max_chunks && threadIdx.x == 0) chunk_used[nx] = 0;
+        }
+        if (threadIdx.x == 0 && w.written) atomicAdd(total, w.written);
     }
 };
+
+// Per chunk: holes below the pair count P and entries at or beyond P (movers).
+__global__ void k_nl_holes(long long nchunks, const int *__restrict__ used, const unsigned long long *__restrict__ total,
+                           long long cap, int *__restrict__ holes, int *__restrict__ movers) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > nchunks) return;
+    const long long P = (long long)*total;
+    if (c == nchunks || P > cap) {
+        holes[c] = 0;
+        movers[c] = 0;
+        return;
+    }
+    const long long start = c * NL_CHUNK, u = used[c];
+    const long long vend = start + u;
+    long long h = 0, m = 0;
+    if (vend < P) h = min(start + NL_CHUNK, P) - vend;
+    if (vend > P) m = vend - max(start, P);
+    holes[c] = (int)h;
+    movers[c] = (int)m;
+}
+
+// Largest k in [lo, hi) with a[k] <= v (non-decreasing a).
+__device__ inline long long last_le64(const int *__restrict__ a, long long lo, long long hi, long long v) {
+    while (hi - lo > 1) {
+        long long mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void k_nl_fill(long long nchunks, const int *__restrict__ used, const int *__restrict__ hstart,
+                          const int *__restrict__ mstart, const unsigned long long *__restrict__ total, NLOut<T> out) {
+    const long long H = hstart[nchunks];
+    const long long P = (long long)*total;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < H; k += (long long)gridDim.x * blockDim.x) {
+        const long long cm = last_le64(mstart, 0, nchunks, k);
+        const long long src = max(cm * NL_CHUNK, P) + (k - mstart[cm]);
+        const long long ch = last_le64(hstart, 0, nchunks, k);
+        const long long dst = ch * NL_CHUNK + used[ch] + (k - hstart[ch]);
+        int i, j;
+        T d;
+        out.get(src, i, j, d);
+        out.put(dst, i, j, d);
+    }
+}
 
 // Soft-core parameters for the device: pair tables of eps_ij and sigma_ij^2 (LB mixing).
 template <typename T>
@@ -355,17 +503,18 @@ struct OpSoftCore {
     int tric;
     int dim;
 
-    __device__ void wave_init(Shared &sh) {
+    struct Wave {};
+    __device__ void wave_init(Shared &sh, Wave &) const {
         if (threadIdx.x < 8) { sh.gde[threadIdx.x] = 0; sh.gds[threadIdx.x] = 0; }
         __syncthreads();
     }
-    __device__ void lane_init(Lane &l, const Ent<T> &p, bool) {
+    __device__ void lane_init(Wave &, Lane &l, const Ent<T> &p, bool) const {
         l.fx = l.fy = l.fz = T(0);
         l.e = T(0);
         l.aeps = l.asig = T(0);
         l.ti = prm.ntypes ? (int)((ent_w(p) >> TYPE_SHIFT) & 7u) : 0;
     }
-    __device__ void candidate(Shared &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+    __device__ void candidate(Shared &, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         if (!ok) return;
         int t = 0;
         if (prm.ntypes) t = l.ti * 8 + (int)((ent_w(q) >> TYPE_SHIFT) & 7u);
@@ -387,7 +536,7 @@ struct OpSoftCore {
             l.asig += e * w / prm.sig[t];
         }
     }
-    __device__ void lane_end(Shared &sh, Lane &l, const Ent<T> &p, bool valid) {
+    __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &p, bool valid) const {
         double ev = valid ? 0.5 * (double)l.e : 0.0;
         ev = wave_sum(ev);
         if (threadIdx.x == 0) atomicAdd(energy, ev);
@@ -408,7 +557,7 @@ struct OpSoftCore {
             }
         }
     }
-    __device__ void wave_end(Shared &sh) {
+    __device__ void wave_end(Shared &sh, Wave &) const {
         __syncthreads();
         if (ADJ && threadIdx.x < 8 && (threadIdx.x < (prm.ntypes ? prm.ntypes : 1))) {
             atomicAdd(&deps[threadIdx.x], sh.gde[threadIdx.x]);
@@ -437,7 +586,8 @@ struct OpHistogram {
     unsigned long long *counts;
     double *vsum;
 
-    __device__ void wave_init(Shared &sh) {
+    struct Wave {};
+    __device__ void wave_init(Shared &sh, Wave &) const {
         for (int k = threadIdx.x; k < nbins; k += 64) { sh.cnt[k] = 0; sh.vs[k] = 0; }
         __syncthreads();
     }
@@ -448,7 +598,7 @@ struct OpHistogram {
             o[0] = v[0]; o[1] = v[1]; o[2] = v[2];
         }
     }
-    __device__ void lane_init(Lane &l, const Ent<T> &p, bool valid) {
+    __device__ void lane_init(Wave &, Lane &l, const Ent<T> &p, bool valid) const {
         l.vx = l.vy = l.vz = T(0);
         if (vel && valid) {
             const uint32_t i = ent_w(p) & IDX_MASK;
@@ -458,7 +608,7 @@ struct OpHistogram {
             l.vx = o[0]; l.vy = o[1]; l.vz = o[2];
         }
     }
-    __device__ void candidate(Shared &sh, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+    __device__ void candidate(Shared &sh, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         if (!ok) return;
         const T d = dev_sqrt_rn(d2);
         T bf = dev_floor(dev_div_rn(d, width));
@@ -473,8 +623,8 @@ struct OpHistogram {
             if (d > T(0)) atomicAdd(&sh.vs[b], (double)(t / d));
         }
     }
-    __device__ void lane_end(Shared &, Lane &, const Ent<T> &, bool) {}
-    __device__ void wave_end(Shared &sh) {
+    __device__ void lane_end(Shared &, Wave &, Lane &, const Ent<T> &, bool) const {}
+    __device__ void wave_end(Shared &sh, Wave &) const {
         __syncthreads();
         for (int k = threadIdx.x; k < nbins; k += 64) {
             if (sh.cnt[k]) atomicAdd(&counts[k], (unsigned long long)sh.cnt[k]);
@@ -501,21 +651,22 @@ struct OpMinDist {
     __host__ __device__ static bool better(T d2, int a, int b, T e2, int ea, int eb) {
         return d2 < e2 || (d2 == e2 && (a < ea || (a == ea && b < eb)));
     }
-    __device__ void wave_init(Shared &sh) {
+    struct Wave {};
+    __device__ void wave_init(Shared &sh, Wave &) const {
         if (threadIdx.x == 0) { sh.d2 = T(INFINITY); sh.a = sh.b = 0x7fffffff; }
         __syncthreads();
     }
-    __device__ void lane_init(Lane &l, const Ent<T> &, bool) {
+    __device__ void lane_init(Wave &, Lane &l, const Ent<T> &, bool) const {
         l.d2 = T(INFINITY);
         l.a = l.b = 0x7fffffff;
     }
-    __device__ void candidate(Shared &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) {
+    __device__ void candidate(Shared &, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         if (!ok) return;
         int i = (int)(ent_w(p) & IDX_MASK), j = (int)(ent_w(q) & IDX_MASK);
         int a = i < j ? i : j, b = i < j ? j : i;
         if (better(d2, a, b, l.d2, l.a, l.b)) { l.d2 = d2; l.a = a; l.b = b; }
     }
-    __device__ void lane_end(Shared &sh, Lane &l, const Ent<T> &, bool) {
+    __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &, bool) const {
         T d2 = l.d2;
         int a = l.a, b = l.b;
         for (int off = 32; off > 0; off >>= 1) {
@@ -527,7 +678,7 @@ struct OpMinDist {
         if (threadIdx.x == 0 && better(d2, a, b, sh.d2, sh.a, sh.b)) { sh.d2 = d2; sh.a = a; sh.b = b; }
         __syncthreads();
     }
-    __device__ void wave_end(Shared &sh) {
+    __device__ void wave_end(Shared &sh, Wave &) const {
         __syncthreads();
         if (threadIdx.x == 0) {
             wd2[blockIdx.x] = sh.d2;

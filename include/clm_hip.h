@@ -113,8 +113,9 @@ clm_status clm_cell_list_info(clm_ctx *ctx, int64_t *out4);
 /* neighborlist!(system) / push_pair!: src/API/neighborlist.jl:217-231,
  * src/internals/neighborlist.jl:67-76.  Writes (i[k], j[k], d[k]) for every pair
  * within the cutoff, d in the context dtype, order unspecified.  If the pair count
- * exceeds capacity, only capacity pairs are written, *npairs holds the required
- * count and CLM_ERR_CAPACITY is returned. */
+ * exceeds capacity, *npairs holds the required count, CLM_ERR_CAPACITY is returned
+ * and the array contents are unspecified (call again with a larger capacity).
+ * At most 2^31-1 pairs per call. */
 clm_status clm_neighborlist(clm_ctx *ctx, int32_t *i, int32_t *j, void *d, int64_t capacity,
                             int64_t *npairs, int on_device);
 
