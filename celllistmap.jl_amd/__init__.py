@@ -9,4 +9,7 @@ from .core import (  # noqa: F401
     SoftCoreEnergyForces, VelocityHistogram, box_info, EnergyGradients, energy_adjoint, neighborlist, neighborlist_, pairwise, pairwise_, update_,
 )
 
+from .batch import BatchedSystems  # noqa: F401
+from . import autotune, workloads  # noqa: F401
+
 __version__ = "0.1.0"

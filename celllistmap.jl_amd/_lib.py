@@ -21,10 +21,11 @@ ORTHORHOMBIC, TRICLINIC, NONPERIODIC = 0, 1, 2
 
 EXPORTS = [
     "clm_create", "clm_destroy", "clm_last_error", "clm_set_stream", "clm_version",
-    "clm_set_box", "clm_get_box", "clm_box_compute",
+    "clm_set_box", "clm_get_box", "clm_get_system_box", "clm_box_compute", "clm_set_systems",
     "clm_set_positions", "clm_set_types", "clm_set_velocities",
     "clm_update_cell_list", "clm_cell_list_info",
-    "clm_neighborlist", "clm_energy_forces", "clm_energy_adjoint", "clm_histogram", "clm_min_distance",
+    "clm_neighborlist", "clm_energy_forces", "clm_energy_forces_batch", "clm_energy_adjoint", "clm_histogram",
+    "clm_min_distance", "clm_graph_capture_energy_forces", "clm_graph_launch", "clm_set_tuning",
     "clm_profile_enable", "clm_profile_get", "clm_profile_reset",
 ]
 
@@ -79,6 +80,12 @@ def lib():
         "clm_version": ([], C.c_char_p),
         "clm_set_box": ([P, P, I, D, I], I),
         "clm_get_box": ([P, C.POINTER(BoxInfo)], I),
+        "clm_get_system_box": ([P, I, C.POINTER(BoxInfo)], I),
+        "clm_set_systems": ([P, I, P, P, I, P, P], I),
+        "clm_energy_forces_batch": ([P, C.POINTER(SoftCore), P, P, I], I),
+        "clm_graph_capture_energy_forces": ([P, C.POINTER(SoftCore), P], I),
+        "clm_graph_launch": ([P, P], I),
+        "clm_set_tuning": ([P, I], I),
         "clm_box_compute": ([I, I, P, I, D, I, P, C.POINTER(BoxInfo)], I),
         "clm_set_positions": ([P, P, I64, I], I),
         "clm_set_types": ([P, P, I64, I], I),

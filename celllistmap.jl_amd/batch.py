@@ -1,0 +1,128 @@
+"""Many independent periodic systems in one device context.
+
+The reference handles one system per ParticleSystem; a user with hundreds of systems
+loops over ``pairwise!`` (src/API/pairwise.jl:48-63).  ``BatchedSystems`` holds all of
+them in one context: one binning pass, one persistent sweep launch over the wave
+tasks of every system (each with its own box, cutoff and lcell), and, optionally, the
+whole update + energy/forces pipeline captured in a HIP graph and replayed.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .core import (ArgumentError, DimensionMismatch, SoftCoreEnergyForces, _Context, _dtype_code, _is_torch,
+                   _np_dtype, _positions, _ptr, _torch, _unitcell)
+
+__all__ = ["BatchedSystems"]
+
+
+class BatchedSystems:
+    """BatchedSystems(positions, unitcells, cutoffs; lcell) -- ``positions`` is a list of
+    (N_k, dim) arrays (or one concatenated array with ``counts``), ``unitcells`` a list
+    of side vectors (orthorhombic) or matrices (triclinic), ``cutoffs`` and ``lcell``
+    scalars or per-system sequences."""
+
+    def __init__(self, positions, unitcells: Sequence, cutoffs, *, counts=None, lcell=1, dtype=None,
+                 device: int = 0):
+        if counts is None:
+            parts = list(positions)
+            counts = [int(np.asarray(p.shape)[0]) for p in parts]
+            if parts and _is_torch(parts[0]):
+                x = _torch().cat([p.reshape(-1, p.shape[-1]) for p in parts])
+            else:
+                x = np.concatenate([np.asarray(p).reshape(len(p), -1) for p in parts]) if parts else np.zeros((0, 3))
+        else:
+            x = positions
+        self.counts = np.asarray(counts, dtype=np.int64)
+        nsys = len(self.counts)
+        if len(unitcells) != nsys:
+            raise DimensionMismatch("one unit cell per system")
+        cut = np.broadcast_to(np.asarray(cutoffs, dtype=np.float64), (nsys,)).copy()
+        lc = np.broadcast_to(np.asarray(lcell, dtype=np.int32), (nsys,)).copy()
+        code = None if dtype is None else _dtype_code(dtype)
+        xs, on_dev, n, dim, code = _positions(x, code)
+        if n != int(self.counts.sum()):
+            raise DimensionMismatch("positions must hold every particle of every system")
+        cts, cells = set(), []
+        for uc in unitcells:
+            ct, cell = _unitcell(uc, dim)
+            cts.add(ct)
+            cells.append(cell)
+        if len(cts) != 1 or _lib.NONPERIODIC in cts:
+            raise ArgumentError("all systems of a batch must share one periodic cell type")
+        self.celltype = cts.pop()
+        self.dim = dim
+        self.nsys = nsys
+        self.dtype = _np_dtype(code)
+        self.cutoffs = cut
+        self.lcells = lc
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)])
+        self._ctx = _Context(code, dim, device)
+        cellsarr = np.ascontiguousarray(np.concatenate(cells))
+        ctx = self._ctx
+        ctx.check(ctx.lib.clm_set_systems(ctx.ptr, nsys, _ptr(self.counts), _ptr(cellsarr), self.celltype,
+                                          _ptr(self.cutoffs), _ptr(self.lcells)), "set_systems")
+        self._cells = cellsarr
+        self.set_positions(xs)
+        self._graph_forces = None
+
+    @property
+    def n(self) -> int:
+        return int(self.offsets[-1])
+
+    def set_positions(self, x):
+        xs, on_dev, n, dim, _ = _positions(x, self._ctx.dtype_code, self.dim)
+        if n != self.n or dim != self.dim:
+            raise DimensionMismatch("positions must hold every particle of every system")
+        if on_dev:
+            self._ctx.use_torch_stream()
+        self._ctx.check(self._ctx.lib.clm_set_positions(self._ctx.ptr, _ptr(xs), int(n), int(on_dev)), "positions")
+        self._xref = xs
+
+    def set_tuning(self, waves_per_cu: int):
+        self._ctx.check(self._ctx.lib.clm_set_tuning(self._ctx.ptr, int(waves_per_cu)), "tuning")
+
+    def system_box(self, k: int) -> dict:
+        from .core import _box_dict
+        out = _lib.BoxInfo()
+        self._ctx.check(self._ctx.lib.clm_get_system_box(self._ctx.ptr, int(k), C.byref(out)), "box")
+        return _box_dict(out)
+
+    def energy_forces(self, eps=0.2, sigma=3.0, device_output: bool = False):
+        """Soft-core energies per system (numpy, float64) and forces of all particles."""
+        p = SoftCoreEnergyForces(eps, sigma).params()
+        e = np.zeros(self.nsys)
+        ctx = self._ctx
+        if device_output:
+            ctx.use_torch_stream()
+            torch = _torch()
+            f = torch.empty((self.n, self.dim), dtype=torch.float32 if self.dtype == np.float32 else torch.float64,
+                            device=f"cuda:{ctx.device}")
+        else:
+            f = np.empty((self.n, self.dim), dtype=self.dtype)
+        ctx.check(ctx.lib.clm_energy_forces_batch(ctx.ptr, C.byref(p), _ptr(e), _ptr(f), int(device_output)),
+                  "energy_forces")
+        return e, f
+
+    def capture(self, eps=0.2, sigma=3.0, forces=None):
+        """Capture update + energy/forces into a HIP graph (forces: optional device tensor)."""
+        p = SoftCoreEnergyForces(eps, sigma).params()
+        ctx = self._ctx
+        ctx.use_torch_stream() if forces is not None else None
+        ctx.check(ctx.lib.clm_update_cell_list(ctx.ptr), "UpdateCellList!")
+        ctx.check(ctx.lib.clm_graph_capture_energy_forces(ctx.ptr, C.byref(p), _ptr(forces)), "graph capture")
+        self._graph_forces = forces
+
+    def replay(self) -> np.ndarray:
+        e = np.zeros(self.nsys)
+        rc = self._ctx.lib.clm_graph_launch(self._ctx.ptr, _ptr(e))
+        self._ctx.check(rc, "graph launch")
+        return e
+
+    def split(self, arr):
+        """Split a per-particle array into per-system pieces."""
+        return [arr[self.offsets[k]:self.offsets[k + 1]] for k in range(self.nsys)]

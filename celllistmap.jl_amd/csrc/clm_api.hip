@@ -43,14 +43,22 @@ struct clm_ctx {
     std::string err;
     int num_cus = 256;
 
-    // geometry
+    // geometry: one entry per system (a single system unless clm_set_systems was used)
     bool has_box = false;
     int celltype = CLM_ORTHORHOMBIC, lcell = 1;
-    double cell_in[9] = {0};
     double cutoff_in = 0;
-    HostBox<float> hbf;
-    HostBox<double> hbd;
-    bool box_built = false;
+    int nsys = 1;
+    std::vector<HostBox<float>> hbf;
+    std::vector<HostBox<double>> hbd;
+    std::vector<int64_t> sys_n;
+    std::vector<int> h_rowoff, h_celloff;
+    bool layout_dirty = true;
+    DBuf sys_desc, sys_of, rowoff;
+    int waves_per_cu = 0;  // persistent sweep grid; 0 = as many as fit
+
+    // captured pipeline (update + energy/forces) for replay
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 
     // particles
     int64_t n = 0;
@@ -181,11 +189,13 @@ clm_status scan_int(clm_ctx *c, const int *in, int *out, int n) {
 }
 
 template <typename T>
-HostBox<T> &hbox(clm_ctx *c);
+std::vector<HostBox<T>> &hboxes(clm_ctx *c);
 template <>
-HostBox<float> &hbox<float>(clm_ctx *c) { return c->hbf; }
+std::vector<HostBox<float>> &hboxes<float>(clm_ctx *c) { return c->hbf; }
 template <>
-HostBox<double> &hbox<double>(clm_ctx *c) { return c->hbd; }
+std::vector<HostBox<double>> &hboxes<double>(clm_ctx *c) { return c->hbd; }
+template <typename T>
+HostBox<T> &hbox(clm_ctx *c) { return hboxes<T>(c)[0]; }
 
 template <typename T>
 DevBox<T> dev_box(const HostBox<T> &hb) {
@@ -291,83 +301,144 @@ clm_status nonperiodic_box(clm_ctx *c) {
         cell[k + D * k] = double((mx[k] - mn[k]) + pad);
         origin[k] = double(mn[k]);
     }
+    hboxes<T>(c).resize(1);
     int rcode = build_host_box<T>(D, cell, CLM_NONPERIODIC, c->cutoff_in, c->lcell, origin, hbox<T>(c));
     return box_status(c, rcode);
 }
 
+// Cell/row offsets of every system and the device system table.
 template <typename T>
-clm_status update_cell_list_t(clm_ctx *c) {
-    if (!c->has_box) return fail(c, CLM_ERR_STATE, "no box set");
-    if (!c->has_pos) return fail(c, CLM_ERR_STATE, "no positions set");
-    if (c->celltype == CLM_NONPERIODIC) {
-        clm_status s = nonperiodic_box<T>(c);
-        if (s) return s;
+clm_status layout_systems(clm_ctx *c) {
+    std::vector<HostBox<T>> &hb = hboxes<T>(c);
+    const int ns = (int)hb.size();
+    c->h_celloff.assign(ns + 1, 0);
+    c->h_rowoff.assign(ns + 1, 0);
+    int64_t cells = 0, rows = 0, pos = 0;
+    std::vector<SysDesc<T>> desc(ns);
+    for (int k = 0; k < ns; ++k) {
+        c->h_celloff[k] = (int)cells;
+        c->h_rowoff[k] = (int)rows;
+        desc[k].box = dev_box(hb[k]);
+        for (int q = 0; q < 9; ++q) desc[k].invrot[q] = hb[k].invrot.a[q];
+        desc[k].cell_off = (int)cells;
+        desc[k].row_off = (int)rows;
+        desc[k].pos_off = (int)pos;
+        desc[k].n = (int)(ns == 1 ? c->n : c->sys_n[k]);
+        cells += (int64_t)hb[k].nc[0] * hb[k].nc[1] * hb[k].nc[2];
+        rows += (int64_t)hb[k].nc[1] * hb[k].nc[2];
+        pos += desc[k].n;
+        if (cells > ((int64_t)1 << 30)) return fail(c, CLM_ERR_ARG, "too many computing cells");
     }
-    HostBox<T> &hb = hbox<T>(c);
-    const int D = c->dim;
-    const int64_t ncells64 = (int64_t)hb.nc[0] * hb.nc[1] * hb.nc[2];
-    if (ncells64 > (int64_t)1 << 30) return fail(c, CLM_ERR_ARG, "too many computing cells");
-    if (c->n >= (int64_t)IDX_MASK) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
-    c->ncells = (int)ncells64;
-    c->nrows = hb.nc[1] * hb.nc[2];
+    c->h_celloff[ns] = (int)cells;
+    c->h_rowoff[ns] = (int)rows;
+    c->ncells = (int)cells;
+    c->nrows = (int)rows;
+    HIPCHK(c, ensure(c->sys_desc, sizeof(SysDesc<T>) * ns));
+    HIPCHK(c, ensure(c->rowoff, sizeof(int) * (ns + 1)));
+    HIPCHK(c, hipMemcpyAsync(c->sys_desc.p, desc.data(), sizeof(SysDesc<T>) * ns, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->rowoff.p, c->h_rowoff.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+    clm_status s = sync(c);
+    if (s) return s;
+    c->layout_dirty = false;
+    return CLM_OK;
+}
+
+template <typename T>
+clm_status ensure_bin_buffers(clm_ctx *c) {
     const int64_t n = c->n;
     const int nc1 = c->ncells + 1;
-    HIPCHK(c, ensure(c->cnt, sizeof(int) * nc1));
-    HIPCHK(c, ensure(c->creal, sizeof(int) * nc1));
-    HIPCHK(c, ensure(c->cstart, sizeof(int) * nc1));
-    HIPCHK(c, ensure(c->cur_real, sizeof(int) * nc1));
-    HIPCHK(c, ensure(c->cur_ghost, sizeof(int) * nc1));
+    for (DBuf *b : {&c->cnt, &c->creal, &c->cstart, &c->cur_real, &c->cur_ghost}) HIPCHK(c, ensure(*b, sizeof(int) * nc1));
     HIPCHK(c, ensure(c->xw, sizeof(Ent<T>) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->home_cell, sizeof(int) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->flags, sizeof(uint32_t) * 4));
+    return CLM_OK;
+}
+
+const int *sys_of_ptr(clm_ctx *c) { return c->nsys > 1 ? (const int *)c->sys_of.p : nullptr; }
+
+// k_bin_count + scan of the cell counts (no host synchronisation).
+template <typename T>
+clm_status enqueue_count(clm_ctx *c) {
+    const int64_t n = c->n;
+    const int nc1 = c->ncells + 1;
     clm_status s;
     if ((s = fill_int(c, c->cnt.p, nc1, 0))) return s;
     if ((s = fill_int(c, c->creal.p, nc1, 0))) return s;
     if ((s = fill_int(c, c->flags.p, 4, 0))) return s;
-    const DevBox<T> db = dev_box(hb);
-    const int periodic = c->celltype != CLM_NONPERIODIC;
+    const SysDesc<T> *sd = (const SysDesc<T> *)c->sys_desc.p;
     if (n > 0) {
-        if (D == 3)
+        if (c->dim == 3)
             LAUNCH(c, "k_bin_count", hipLaunchKernelGGL((k_bin_count<T, 3>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
-                                                        (const T *)c->pos.p, (int)n, db, periodic, (Ent<T> *)c->xw.p,
+                                                        (const T *)c->pos.p, (int)n, sd, sys_of_ptr(c), (Ent<T> *)c->xw.p,
                                                         (int *)c->home_cell.p, (int *)c->cnt.p, (int *)c->creal.p,
                                                         (uint32_t *)c->flags.p));
         else
             LAUNCH(c, "k_bin_count", hipLaunchKernelGGL((k_bin_count<T, 2>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
-                                                        (const T *)c->pos.p, (int)n, db, periodic, (Ent<T> *)c->xw.p,
+                                                        (const T *)c->pos.p, (int)n, sd, sys_of_ptr(c), (Ent<T> *)c->xw.p,
                                                         (int *)c->home_cell.p, (int *)c->cnt.p, (int *)c->creal.p,
                                                         (uint32_t *)c->flags.p));
     }
-    if ((s = scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1))) return s;
-    int total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    if ((s = sync(c))) return s;
-    if (*c->h_flags & FLAG_NAN) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN or Inf)");
-    if (*c->h_flags & FLAG_RANGE)
-        return fail(c, CLM_ERR_RANGE, "a particle maps outside the computing grid (the reference raises a BoundsError)");
-    c->n_entries = total;
-    HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
-    c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
+    return scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1);
+}
+
+// k_bin_scatter into E (capacity c->e_cap; overflow raises FLAG_OVERFLOW).
+template <typename T>
+clm_status enqueue_scatter(clm_ctx *c) {
+    const int64_t n = c->n;
+    const int nc1 = c->ncells + 1;
+    clm_status s;
     if ((s = fill_int(c, c->cur_real.p, nc1, 0))) return s;
     if ((s = fill_int(c, c->cur_ghost.p, nc1, 0))) return s;
+    const SysDesc<T> *sd = (const SysDesc<T> *)c->sys_desc.p;
     if (n > 0) {
         const int *tp = c->has_types ? (const int *)c->types.p : nullptr;
-        if (D == 3)
+        if (c->dim == 3)
             LAUNCH(c, "k_bin_scatter", hipLaunchKernelGGL((k_bin_scatter<T, 3>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
-                                                          (int)n, db, periodic, (const Ent<T> *)c->xw.p,
+                                                          (int)n, sd, sys_of_ptr(c), (const Ent<T> *)c->xw.p,
                                                           (const int *)c->home_cell.p, (const int *)c->cstart.p,
                                                           (const int *)c->creal.p, tp, (int *)c->cur_real.p,
                                                           (int *)c->cur_ghost.p, (Ent<T> *)c->E.p, c->e_cap,
                                                           (uint32_t *)c->flags.p));
         else
             LAUNCH(c, "k_bin_scatter", hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
-                                                          (int)n, db, periodic, (const Ent<T> *)c->xw.p,
+                                                          (int)n, sd, sys_of_ptr(c), (const Ent<T> *)c->xw.p,
                                                           (const int *)c->home_cell.p, (const int *)c->cstart.p,
                                                           (const int *)c->creal.p, tp, (int *)c->cur_real.p,
                                                           (int *)c->cur_ghost.p, (Ent<T> *)c->E.p, c->e_cap,
                                                           (uint32_t *)c->flags.p));
     }
+    return CLM_OK;
+}
+
+clm_status check_flags(clm_ctx *c, uint32_t fl) {
+    if (fl & FLAG_NAN) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN or Inf)");
+    if (fl & FLAG_RANGE)
+        return fail(c, CLM_ERR_RANGE, "a particle maps outside the computing grid (the reference raises a BoundsError)");
+    return CLM_OK;
+}
+
+template <typename T>
+clm_status update_cell_list_t(clm_ctx *c) {
+    if (!c->has_box) return fail(c, CLM_ERR_STATE, "no box set");
+    if (!c->has_pos) return fail(c, CLM_ERR_STATE, "no positions set");
+    if (c->n >= (int64_t)IDX_MASK) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
+    clm_status s;
+    if (c->celltype == CLM_NONPERIODIC) {
+        if ((s = nonperiodic_box<T>(c))) return s;
+        c->layout_dirty = true;
+    }
+    if (c->layout_dirty && (s = layout_systems<T>(c))) return s;
+    if ((s = ensure_bin_buffers<T>(c))) return s;
+    if ((s = enqueue_count<T>(c))) return s;
+    int total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync(c))) return s;
+    if ((s = check_flags(c, *c->h_flags))) return s;
+    c->n_entries = total;
+    HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
+    c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
+    if ((s = enqueue_scatter<T>(c))) return s;
     c->cl_valid = true;
     c->dirty = false;
     for (bool &m : c->mode_valid) m = false;
@@ -381,8 +452,9 @@ clm_status ensure_cell_list(clm_ctx *c) {
 
 int mode_a(const clm_ctx *c) { return c->celltype == CLM_TRICLINIC ? MODE_A_TRIC : MODE_A_ORTHO; }
 
-clm_status prepare_mode(clm_ctx *c, int mode) {
-    if (c->mode_valid[mode]) return CLM_OK;
+// Per-cell home counts and per-row wave tasks of a sweep mode (no host synchronisation).
+template <typename T>
+clm_status enqueue_mode(clm_ctx *c, int mode) {
     const int nc1 = c->ncells + 1;
     HIPCHK(c, ensure(c->hc, sizeof(int) * nc1));
     HIPCHK(c, ensure(c->hstart[mode], sizeof(int) * nc1));
@@ -394,37 +466,41 @@ clm_status prepare_mode(clm_ctx *c, int mode) {
                                                   (int *)c->hc.p));
     clm_status s;
     if ((s = scan_int(c, (const int *)c->hc.p, (int *)c->hstart[mode].p, nc1))) return s;
-    const int ncx = (c->dtype == CLM_F32) ? c->hbf.nc[0] : c->hbd.nc[0];
-    LAUNCH(c, "k_row_tasks", hipLaunchKernelGGL(k_row_tasks, dim3(blocks(c->nrows + 1, 256)), dim3(256), 0, c->stream,
-                                                c->nrows, ncx, (const int *)c->hstart[mode].p, (int *)c->ntask.p));
-    if ((s = scan_int(c, (const int *)c->ntask.p, (int *)c->tstart[mode].p, c->nrows + 1))) return s;
+    LAUNCH(c, "k_row_tasks", hipLaunchKernelGGL((k_row_tasks<T>), dim3(blocks(c->nrows + 1, 256)), dim3(256), 0, c->stream,
+                                                c->nrows, c->nsys, (const SysDesc<T> *)c->sys_desc.p,
+                                                (const int *)c->rowoff.p, (const int *)c->hstart[mode].p,
+                                                (int *)c->ntask.p));
+    return scan_int(c, (const int *)c->ntask.p, (int *)c->tstart[mode].p, c->nrows + 1);
+}
+
+clm_status prepare_mode(clm_ctx *c, int mode) {
+    if (c->mode_valid[mode]) return CLM_OK;
+    clm_status s = c->dtype == CLM_F32 ? enqueue_mode<float>(c, mode) : enqueue_mode<double>(c, mode);
+    if (s) return s;
     c->mode_valid[mode] = true;
     return CLM_OK;
 }
 
 template <typename T>
 SweepArgs<T> sweep_args(clm_ctx *c, int mode) {
-    const HostBox<T> &hb = hbox<T>(c);
     SweepArgs<T> sa;
-    sa.E = (const Ent<T> *)c->E.p;
-    sa.cstart = (const int *)c->cstart.p;
     sa.hstart = (const int *)c->hstart[mode].p;
     sa.tstart = (const int *)c->tstart[mode].p;
+    sa.rowoff = (const int *)c->rowoff.p;
+    sa.sys = (const SysDesc<T> *)c->sys_desc.p;
     sa.task_counter = (int *)c->counters.p;
-    for (int k = 0; k < 3; ++k) sa.nc[k] = hb.nc[k];
     sa.nrows = c->nrows;
-    sa.lcell = hb.lcell;
-    sa.dim = hb.dim;
-    sa.rc2 = hb.cutoff_sqr;
+    sa.nsys = c->nsys;
     return sa;
 }
 
-// Persistent grid: as many one-wave workgroups as can be resident on every CU.
+// Persistent grid: as many one-wave workgroups per CU as fit (or the tuned number).
 template <typename KernelT>
 int occupancy_grid(clm_ctx *c, KernelT kernel) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess || per_cu <= 0) per_cu = 16;
     per_cu = std::min(per_cu, 32);
+    if (c->waves_per_cu > 0) per_cu = std::min(per_cu, c->waves_per_cu);
     return std::max(64, c->num_cus * per_cu);
 }
 int sweep_grid(clm_ctx *c) { return std::max(64, c->num_cus * 16); }
@@ -436,10 +512,12 @@ clm_status run_sweep(clm_ctx *c, const char *name, const Op &op, int grid) {
     if ((s = fill_int(c, c->counters.p, 16, 0))) return s;
     SweepArgs<T> sa = sweep_args<T>(c, MODE);
     if (grid <= 0) grid = c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE, 2, Op>);
+    const Ent<T> *E = (const Ent<T> *)c->E.p;
+    const int *cs = (const int *)c->cstart.p;
     if (c->dim == 3)
-        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, sa.E, sa.cstart, sa, op));
+        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, sa, op));
     else
-        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, sa.E, sa.cstart, sa, op));
+        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, sa, op));
     return CLM_OK;
 }
 
@@ -542,13 +620,38 @@ SoftParams<T> soft_params(const clm_softcore *p) {
     return sp;
 }
 
-template <typename T, bool FORCES, bool ADJ>
-clm_status softcore_t(clm_ctx *c, const clm_softcore *p, double *energy, void *fout, double *deps, double *dsig,
-                      int on_device, int64_t *npairs) {
-    clm_status s;
+clm_status check_softcore(clm_ctx *c, const clm_softcore *p) {
     if (!p) return fail(c, CLM_ERR_ARG, "NULL soft-core parameters");
     if (p->ntypes < 0 || p->ntypes > 8) return fail(c, CLM_ERR_ARG, "ntypes must be in 0..8");
     if (p->ntypes > 0 && !c->has_types) return fail(c, CLM_ERR_STATE, "typed parameters need clm_set_types");
+    return CLM_OK;
+}
+
+// Launch the soft-core sweep (mode B) writing forces/gradients to df and the
+// per-system energies + per-type gradients to c->acc (no host synchronisation).
+template <typename T, bool FORCES, bool ADJ>
+clm_status enqueue_softcore(clm_ctx *c, const clm_softcore *p, T *df) {
+    clm_status s;
+    const int nacc = c->nsys + 16;
+    HIPCHK(c, ensure(c->acc, sizeof(double) * nacc));
+    if ((s = fill_int(c, c->acc.p, 2 * nacc, 0))) return s;
+    OpSoftCore<T, FORCES, ADJ> op;
+    std::memset((void *)&op, 0, sizeof(op));
+    op.prm = soft_params<T>(p);
+    op.forces = df;
+    op.energy = (double *)c->acc.p;
+    op.deps = (double *)c->acc.p + c->nsys;
+    op.dsig = (double *)c->acc.p + c->nsys + 8;
+    op.dim = c->dim;
+    const char *name = ADJ ? "k_sweep_energy_adjoint" : (FORCES ? "k_sweep_energy_forces" : "k_sweep_energy");
+    return run_sweep<T, MODE_B>(c, name, op, 0);
+}
+
+template <typename T, bool FORCES, bool ADJ>
+clm_status softcore_t(clm_ctx *c, const clm_softcore *p, double *energies, int per_system, void *fout, double *deps,
+                      double *dsig, int on_device) {
+    clm_status s;
+    if ((s = check_softcore(c, p))) return s;
     if ((s = ensure_cell_list(c))) return s;
     if ((s = prepare_mode(c, MODE_B))) return s;
     const int D = c->dim;
@@ -557,34 +660,97 @@ clm_status softcore_t(clm_ctx *c, const clm_softcore *p, double *energy, void *f
         HIPCHK(c, ensure(c->outbuf, sizeof(T) * std::max<int64_t>(c->n * D, 1)));
         df = (T *)c->outbuf.p;
     }
-    HIPCHK(c, ensure(c->acc, sizeof(double) * 32));
-    if ((s = fill_int(c, c->acc.p, 64, 0))) return s;
-    OpSoftCore<T, FORCES, ADJ> op;
-    op.prm = soft_params<T>(p);
-    op.forces = df;
-    op.energy = (double *)c->acc.p;
-    op.deps = (double *)c->acc.p + 8;
-    op.dsig = (double *)c->acc.p + 16;
-    const HostBox<T> &hb = hbox<T>(c);
-    for (int k = 0; k < 9; ++k) op.invrot[k] = hb.invrot.a[k];
-    op.tric = c->celltype == CLM_TRICLINIC;
-    op.dim = D;
-    const char *name = ADJ ? "k_sweep_energy_adjoint" : (FORCES ? "k_sweep_energy_forces" : "k_sweep_energy");
-    if ((s = run_sweep<T, MODE_B>(c, name, op, 0))) return s;
-    double acc[24];
-    HIPCHK(c, hipMemcpyAsync(acc, c->acc.p, sizeof(double) * 24, hipMemcpyDeviceToHost, c->stream));
+    if ((s = enqueue_softcore<T, FORCES, ADJ>(c, p, df))) return s;
+    std::vector<double> acc(c->nsys + 16);
+    HIPCHK(c, hipMemcpyAsync(acc.data(), c->acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, c->stream));
     if ((FORCES || ADJ) && fout && !on_device && c->n > 0)
         HIPCHK(c, hipMemcpyAsync(fout, df, sizeof(T) * c->n * D, hipMemcpyDeviceToHost, c->stream));
     if ((s = sync(c))) return s;
-    if (energy) *energy = acc[0];
+    if (energies) {
+        if (per_system) {
+            for (int k = 0; k < c->nsys; ++k) energies[k] = acc[k];
+        } else {
+            double e = 0;
+            for (int k = 0; k < c->nsys; ++k) e += acc[k];
+            *energies = e;
+        }
+    }
     const int nt = p->ntypes > 0 ? p->ntypes : 1;
     if (ADJ) {
         for (int k = 0; k < nt; ++k) {
-            if (deps) deps[k] = acc[8 + k];
-            if (dsig) dsig[k] = acc[16 + k];
+            if (deps) deps[k] = acc[c->nsys + k];
+            if (dsig) dsig[k] = acc[c->nsys + 8 + k];
         }
     }
-    if (npairs) *npairs = -1;
+    return CLM_OK;
+}
+
+// Records update + energy/forces (device forces array) into a HIP graph.  The cell list
+// must have been built once eagerly (it sizes the entry buffer with headroom); a replay
+// whose entries exceed that capacity reports CLM_ERR_CAPACITY and must be re-captured.
+template <typename T>
+clm_status graph_capture_t(clm_ctx *c, const clm_softcore *p, void *forces) {
+    clm_status s;
+    if ((s = check_softcore(c, p))) return s;
+    if (c->celltype == CLM_NONPERIODIC) return fail(c, CLM_ERR_ARG, "graph capture needs a periodic box");
+    if ((s = ensure_cell_list(c))) return s;
+    // headroom for fluctuating ghost counts between replays
+    const int64_t want = c->n_entries + c->n_entries / 8 + 1024;
+    if (c->e_cap < want) {
+        HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * (want + SWEEP_PAD)));
+        c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
+    }
+    const int64_t nD = std::max<int64_t>(c->n * c->dim, 1);
+    HIPCHK(c, ensure(c->outbuf, sizeof(T) * nD));
+    HIPCHK(c, ensure(c->hc, sizeof(int) * (c->ncells + 1)));
+    HIPCHK(c, ensure(c->hstart[MODE_B], sizeof(int) * (c->ncells + 1)));
+    HIPCHK(c, ensure(c->ntask, sizeof(int) * (c->nrows + 1)));
+    HIPCHK(c, ensure(c->tstart[MODE_B], sizeof(int) * (c->nrows + 1)));
+    HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
+    HIPCHK(c, ensure(c->acc, sizeof(double) * (c->nsys + 16)));
+    {  // size the scan scratch for the largest scan before capturing
+        size_t need = 0;
+        hipcub::DeviceScan::ExclusiveSum(nullptr, need, (const int *)nullptr, (int *)nullptr, c->ncells + 1, c->stream);
+        HIPCHK(c, ensure(c->scan_tmp, need));
+    }
+    if (c->graph_exec) { hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    const bool prof = c->prof;
+    c->prof = false;  // no events inside the capture
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    clm_status cs = enqueue_count<T>(c);
+    if (!cs) cs = enqueue_scatter<T>(c);
+    if (!cs) cs = enqueue_mode<T>(c, MODE_B);
+    T *df = forces ? (T *)forces : (T *)c->outbuf.p;
+    if (!cs) cs = enqueue_softcore<T, true, false>(c, p, df);
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    c->prof = prof;
+    if (cs) return cs;
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    return CLM_OK;
+}
+
+clm_status graph_launch(clm_ctx *c, double *energies) {
+    if (!c->graph_exec) return fail(c, CLM_ERR_STATE, "no captured graph");
+    HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    std::vector<double> acc(c->nsys);
+    int total = 0;
+    HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (energies) HIPCHK(c, hipMemcpyAsync(acc.data(), c->acc.p, sizeof(double) * c->nsys, hipMemcpyDeviceToHost, c->stream));
+    clm_status s = sync(c);
+    if (s) return s;
+    if ((s = check_flags(c, *c->h_flags))) return s;
+    if (total > c->e_cap || (*c->h_flags & FLAG_OVERFLOW))
+        return fail(c, CLM_ERR_CAPACITY, "entries exceed the captured capacity; capture again");
+    c->n_entries = total;
+    c->cl_valid = true;
+    c->dirty = false;
+    for (bool &m : c->mode_valid) m = false;
+    c->mode_valid[MODE_B] = true;
+    if (energies)
+        for (int k = 0; k < c->nsys; ++k) energies[k] = acc[k];
     return CLM_OK;
 }
 
@@ -601,6 +767,7 @@ clm_status histogram_t(clm_ctx *c, int nbins, double width, int64_t *counts, dou
     HIPCHK(c, ensure(c->hist_v, sizeof(double) * nbins));
     if ((s = fill_int(c, c->hist_c.p, 2 * nbins, 0))) return s;
     if ((s = fill_int(c, c->hist_v.p, 2 * nbins, 0))) return s;
+    if (c->nsys != 1) return fail(c, CLM_ERR_ARG, "histograms are per system (nsys must be 1)");
     OpHistogram<T> op;
     op.nbins = nbins;
     op.width = T(width);
@@ -623,6 +790,7 @@ clm_status histogram_t(clm_ctx *c, int nbins, double width, int64_t *counts, dou
 template <typename T>
 clm_status mindist_t(clm_ctx *c, int64_t *oi, int64_t *oj, double *od) {
     clm_status s;
+    if (c->nsys != 1) return fail(c, CLM_ERR_ARG, "minimum distance is per system (nsys must be 1)");
     if ((s = ensure_cell_list(c))) return s;
     const int ma = mode_a(c);
     if ((s = prepare_mode(c, ma))) return s;
@@ -697,11 +865,14 @@ clm_status clm_destroy(clm_ctx *c) {
                     &c->hstart[1], &c->hstart[2], &c->tstart[0], &c->tstart[1], &c->tstart[2], &c->counters,
                     &c->scan_tmp, &c->nl_i, &c->nl_j, &c->nl_d, &c->nl_count, &c->outbuf, &c->acc,
                     &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b, &c->sp_i, &c->sp_j, &c->sp_d,
-                    &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart};
+                    &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart, &c->sys_desc,
+                    &c->sys_of, &c->rowoff};
     for (DBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
     if (c->h_flags) hipHostFree(c->h_flags);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -757,35 +928,101 @@ clm_status clm_set_box(clm_ctx *c, const double *cell, int celltype, double cuto
     const int D = c->dim;
     if (celltype != CLM_NONPERIODIC) {
         if (!cell) return fail(c, CLM_ERR_ARG, "NULL unit cell");
-        int rc = c->dtype == CLM_F32 ? build_host_box<float>(D, cell, celltype, cutoff, lcell, nullptr, c->hbf)
-                                     : build_host_box<double>(D, cell, celltype, cutoff, lcell, nullptr, c->hbd);
+        HostBox<float> bf;
+        HostBox<double> bd;
+        int rc = c->dtype == CLM_F32 ? build_host_box<float>(D, cell, celltype, cutoff, lcell, nullptr, bf)
+                                     : build_host_box<double>(D, cell, celltype, cutoff, lcell, nullptr, bd);
         clm_status s = box_status(c, rc);
         if (s) return s;
-        for (int k = 0; k < D * D; ++k) c->cell_in[k] = cell[k];
+        c->hbf.assign(1, bf);
+        c->hbd.assign(1, bd);
     }
+    c->nsys = 1;
+    c->sys_n.clear();
     c->celltype = celltype;
     c->cutoff_in = cutoff;
     c->lcell = lcell;
     c->has_box = true;
     c->dirty = true;
+    c->layout_dirty = true;
+    return CLM_OK;
+}
+
+clm_status clm_set_systems(clm_ctx *c, int nsys, const int64_t *n, const double *cells, int celltype,
+                           const double *cutoffs, const int *lcells) {
+    if (!c) return CLM_ERR_ARG;
+    if (nsys < 1 || !n || !cells || !cutoffs || !lcells) return fail(c, CLM_ERR_ARG, "bad system description");
+    if (celltype != CLM_ORTHORHOMBIC && celltype != CLM_TRICLINIC)
+        return fail(c, CLM_ERR_ARG, "systems of a batch must be periodic (orthorhombic or triclinic)");
+    const int D = c->dim;
+    std::vector<HostBox<float>> vf(nsys);
+    std::vector<HostBox<double>> vd(nsys);
+    int64_t total = 0;
+    for (int k = 0; k < nsys; ++k) {
+        if (n[k] < 0) return fail(c, CLM_ERR_ARG, "negative particle count");
+        if (lcells[k] < 1) return fail(c, CLM_ERR_ARG, "lcell must be greater or equal to 1");
+        const double *cell = cells + (size_t)k * D * D;
+        int rc = c->dtype == CLM_F32 ? build_host_box<float>(D, cell, celltype, cutoffs[k], lcells[k], nullptr, vf[k])
+                                     : build_host_box<double>(D, cell, celltype, cutoffs[k], lcells[k], nullptr, vd[k]);
+        clm_status s = box_status(c, rc);
+        if (s) {
+            c->err = "system " + std::to_string(k) + ": " + c->err;
+            return s;
+        }
+        total += n[k];
+    }
+    if (total >= (int64_t)IDX_MASK) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
+    std::vector<int> sof(std::max<int64_t>(total, 1));
+    for (int k = 0, o = 0; k < nsys; ++k)
+        for (int64_t q = 0; q < n[k]; ++q) sof[o++] = k;
+    HIPCHK(c, ensure(c->sys_of, sizeof(int) * sof.size()));
+    HIPCHK(c, hipMemcpyAsync(c->sys_of.p, sof.data(), sizeof(int) * sof.size(), hipMemcpyHostToDevice, c->stream));
+    clm_status s = sync(c);
+    if (s) return s;
+    c->hbf = vf;
+    c->hbd = vd;
+    c->nsys = nsys;
+    c->sys_n.assign(n, n + nsys);
+    c->celltype = celltype;
+    c->cutoff_in = cutoffs[0];
+    c->lcell = lcells[0];
+    c->has_box = true;
+    c->dirty = true;
+    c->layout_dirty = true;
     return CLM_OK;
 }
 
 clm_status clm_get_box(clm_ctx *c, clm_box_info *out) {
+    return clm_get_system_box(c, 0, out);
+}
+
+clm_status clm_get_system_box(clm_ctx *c, int k, clm_box_info *out) {
     if (!c || !out) return CLM_ERR_ARG;
     if (!c->has_box) return fail(c, CLM_ERR_STATE, "no box set");
+    if (k < 0 || k >= c->nsys) return fail(c, CLM_ERR_ARG, "system index out of range");
     if (c->celltype == CLM_NONPERIODIC) {
         clm_status s = ensure_cell_list(c);
         if (s) return s;
     }
-    if (c->dtype == CLM_F32) box_info(c->hbf, out);
-    else box_info(c->hbd, out);
+    if (c->dtype == CLM_F32) box_info(c->hbf[k], out);
+    else box_info(c->hbd[k], out);
+    return CLM_OK;
+}
+
+clm_status clm_set_tuning(clm_ctx *c, int waves_per_cu) {
+    if (!c || waves_per_cu < 0) return CLM_ERR_ARG;
+    c->waves_per_cu = waves_per_cu;
     return CLM_OK;
 }
 
 clm_status clm_set_positions(clm_ctx *c, const void *x, int64_t n, int on_device) {
     if (!c) return CLM_ERR_ARG;
     if (n < 0 || (n > 0 && !x)) return fail(c, CLM_ERR_ARG, "bad positions");
+    if (c->nsys > 1) {
+        int64_t tot = 0;
+        for (int64_t v : c->sys_n) tot += v;
+        if (n != tot) return fail(c, CLM_ERR_DIM, "positions must hold every particle of every system");
+    }
     hipSetDevice(c->device);
     const size_t es = c->dtype == CLM_F32 ? 4 : 8;
     const size_t bytes = es * (size_t)c->dim * (size_t)n;
@@ -796,7 +1033,10 @@ clm_status clm_set_positions(clm_ctx *c, const void *x, int64_t n, int on_device
         clm_status s = sync(c);
         if (s) return s;
     }
-    if (n != c->n) c->has_types = false;
+    if (n != c->n) {
+        c->has_types = false;
+        c->layout_dirty = true;
+    }
     c->n = n;
     c->has_pos = true;
     c->dirty = true;
@@ -873,19 +1113,42 @@ clm_status clm_neighborlist(clm_ctx *c, int32_t *i, int32_t *j, void *d, int64_t
 clm_status clm_energy_forces(clm_ctx *c, const clm_softcore *p, double *energy, void *forces, int on_device, int64_t *npairs) {
     if (!c) return CLM_ERR_ARG;
     hipSetDevice(c->device);
+    if (npairs) *npairs = -1;
     if (forces)
-        return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energy, forces, nullptr, nullptr, on_device, npairs)
-                                   : softcore_t<double, true, false>(c, p, energy, forces, nullptr, nullptr, on_device, npairs);
-    return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energy, nullptr, nullptr, nullptr, on_device, npairs)
-                               : softcore_t<double, false, false>(c, p, energy, nullptr, nullptr, nullptr, on_device, npairs);
+        return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energy, 0, forces, nullptr, nullptr, on_device)
+                                   : softcore_t<double, true, false>(c, p, energy, 0, forces, nullptr, nullptr, on_device);
+    return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energy, 0, nullptr, nullptr, nullptr, on_device)
+                               : softcore_t<double, false, false>(c, p, energy, 0, nullptr, nullptr, nullptr, on_device);
+}
+
+clm_status clm_energy_forces_batch(clm_ctx *c, const clm_softcore *p, double *energies, void *forces, int on_device) {
+    if (!c || !energies) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    if (forces)
+        return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energies, 1, forces, nullptr, nullptr, on_device)
+                                   : softcore_t<double, true, false>(c, p, energies, 1, forces, nullptr, nullptr, on_device);
+    return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energies, 1, nullptr, nullptr, nullptr, on_device)
+                               : softcore_t<double, false, false>(c, p, energies, 1, nullptr, nullptr, nullptr, on_device);
 }
 
 clm_status clm_energy_adjoint(clm_ctx *c, const clm_softcore *p, double *energy, void *dedx, double *deps, double *dsig,
                               int on_device) {
     if (!c) return CLM_ERR_ARG;
     hipSetDevice(c->device);
-    return c->dtype == CLM_F32 ? softcore_t<float, true, true>(c, p, energy, dedx, deps, dsig, on_device, nullptr)
-                               : softcore_t<double, true, true>(c, p, energy, dedx, deps, dsig, on_device, nullptr);
+    return c->dtype == CLM_F32 ? softcore_t<float, true, true>(c, p, energy, 0, dedx, deps, dsig, on_device)
+                               : softcore_t<double, true, true>(c, p, energy, 0, dedx, deps, dsig, on_device);
+}
+
+clm_status clm_graph_capture_energy_forces(clm_ctx *c, const clm_softcore *p, void *forces) {
+    if (!c) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    return c->dtype == CLM_F32 ? graph_capture_t<float>(c, p, forces) : graph_capture_t<double>(c, p, forces);
+}
+
+clm_status clm_graph_launch(clm_ctx *c, double *energies) {
+    if (!c) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    return graph_launch(c, energies);
 }
 
 clm_status clm_histogram(clm_ctx *c, int nbins, double width, int64_t *counts, double *vsum) {

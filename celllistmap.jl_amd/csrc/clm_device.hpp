@@ -72,6 +72,22 @@ __device__ inline T dev_div_rn(T a, T b) { return a / b; }
 template <typename T>
 __device__ inline bool dev_finite(T v) { return isfinite(v); }
 
+// Reciprocal for tolerance-level arithmetic (energies, forces): the hardware estimate
+// refined by Newton steps -- one for float (~1 ulp), two for double (~1 ulp).
+template <typename T>
+__device__ inline T fast_rcp(T x);
+template <>
+__device__ inline float fast_rcp<float>(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+template <>
+__device__ inline double fast_rcp<double>(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
 // Wrapping to the first cell followed by the rotation (CellOperations.jl:56-94,
 // CellLists.jl:944-945), 3x3 Cramer solve in StaticArrays order.  The translation
 // unit is compiled with -ffp-contract=off so every product and sum rounds on its own.

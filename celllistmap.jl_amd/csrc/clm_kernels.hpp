@@ -33,13 +33,31 @@ enum { NONPER_FLAG = 2 };
 // binning
 // ------------------------------------------------------------------------------
 
+// Per-system description for the device: the box, the inverse rotation used for
+// outputs, and where the system's cells, rows and particles start in the global arrays.
+// A single system is a table with one entry; a batch of independent systems (each with
+// its own box, cutoff and lcell) is one table, processed by the same kernels.
+template <typename T>
+struct SysDesc {
+    DevBox<T> box;
+    T invrot[9];
+    int cell_off;
+    int row_off;
+    int pos_off;
+    int n;
+};
+
 template <typename T, int D>
-__global__ void __launch_bounds__(256) k_bin_count(const T *__restrict__ x, int n, DevBox<T> b, int periodic,
-                                                   Ent<T> *__restrict__ xw, int *__restrict__ home_cell,
-                                                   int *__restrict__ cnt, int *__restrict__ creal,
-                                                   uint32_t *__restrict__ gflags) {
+__global__ void __launch_bounds__(256) k_bin_count(const T *__restrict__ x, int n, const SysDesc<T> *__restrict__ sys,
+                                                   const int *__restrict__ sys_of, Ent<T> *__restrict__ xw,
+                                                   int *__restrict__ home_cell, int *__restrict__ cnt,
+                                                   int *__restrict__ creal, uint32_t *__restrict__ gflags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int si = sys_of ? sys_of[i] : 0;
+    const DevBox<T> &b = sys[si].box;
+    const int coff = sys[si].cell_off;
+    const bool periodic = b.celltype != NONPER_FLAG;
     T xi[3] = {T(0), T(0), T(0)};
     bool finite = true;
     for (int k = 0; k < D; ++k) {
@@ -63,7 +81,7 @@ __global__ void __launch_bounds__(256) k_bin_count(const T *__restrict__ x, int 
     e.x = p[0]; e.y = p[1]; e.z = p[2];
     e.w = (uint32_t)i;
     xw[i] = e;
-    int c = cell_of(b, p, true, flags);
+    const int c = coff + cell_of(b, p, true, flags);
     home_cell[i] = c;
     atomicAdd(&cnt[c], 1);
     atomicAdd(&creal[c], 1);
@@ -74,7 +92,7 @@ __global__ void __launch_bounds__(256) k_bin_count(const T *__restrict__ x, int 
             T q[3] = {p[0], p[1], p[2]};
             for (int k = 0; k < D; ++k) q[k] = p[k] + b.shift[m][k];
             if (!in_box(b, q)) continue;
-            int cj = cell_of(b, q, false, flags);
+            const int cj = coff + cell_of(b, q, false, flags);
             atomicAdd(&cnt[cj], 1);
         }
     }
@@ -82,7 +100,8 @@ __global__ void __launch_bounds__(256) k_bin_count(const T *__restrict__ x, int 
 }
 
 template <typename T, int D>
-__global__ void __launch_bounds__(256) k_bin_scatter(int n, DevBox<T> b, int periodic, const Ent<T> *__restrict__ xw,
+__global__ void __launch_bounds__(256) k_bin_scatter(int n, const SysDesc<T> *__restrict__ sys,
+                                                     const int *__restrict__ sys_of, const Ent<T> *__restrict__ xw,
                                                      const int *__restrict__ home_cell, const int *__restrict__ cstart,
                                                      const int *__restrict__ creal, const int *__restrict__ types,
                                                      int *__restrict__ cur_real, int *__restrict__ cur_ghost,
@@ -91,6 +110,9 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, DevBox<T> b, int per
     if (i >= n) return;
     int c = home_cell[i];
     if (c < 0) return;
+    const int si = sys_of ? sys_of[i] : 0;
+    const DevBox<T> &b = sys[si].box;
+    const int coff = sys[si].cell_off;
     uint32_t flags = 0;
     Ent<T> e = xw[i];
     uint32_t tbits = types ? ((uint32_t)types[i] & 7u) << TYPE_SHIFT : 0u;
@@ -99,7 +121,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, DevBox<T> b, int per
     int64_t pos = (int64_t)cstart[c] + slot;
     if (pos < ecap) E[pos] = e;
     else flags |= FLAG_OVERFLOW;
-    if (periodic) {
+    if (b.celltype != NONPER_FLAG) {
         T p[3] = {e.x, e.y, e.z};
         const int nimg = D == 3 ? 27 : 9;
         for (int m = 0; m < nimg; ++m) {
@@ -108,7 +130,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, DevBox<T> b, int per
             for (int k = 0; k < D; ++k) q[k] = p[k] + b.shift[m][k];
             if (!in_box(b, q)) continue;
             uint32_t fl2 = 0;
-            int cj = cell_of(b, q, false, fl2);
+            int cj = coff + cell_of(b, q, false, fl2);
             int s2 = atomicAdd(&cur_ghost[cj], 1);
             int64_t pj = (int64_t)cstart[cj] + creal[cj] + s2;
             Ent<T> g;
@@ -131,11 +153,26 @@ __global__ void k_home_counts(int ncells, int all_entries, const int *__restrict
     hc[c] = all_entries ? (nr > 0 ? cstart[c + 1] - cstart[c] : 0) : nr;
 }
 
-__global__ void k_row_tasks(int nrows, int ncx, const int *__restrict__ hstart, int *__restrict__ ntask) {
+// Largest k in [lo, hi) with a[k] <= v (a non-decreasing, a[lo] <= v).
+__device__ inline int last_le(const int *__restrict__ a, int lo, int hi, int v) {
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void k_row_tasks(int nrows, int nsys, const SysDesc<T> *__restrict__ sys, const int *__restrict__ rowoff,
+                            const int *__restrict__ hstart, int *__restrict__ ntask) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r > nrows) return;
     if (r == nrows) { ntask[r] = 0; return; }
-    int h = hstart[(r + 1) * ncx] - hstart[r * ncx];
+    const int s = last_le(rowoff, 0, nsys + 1, r);
+    const int ncx = sys[s].box.nc[0];
+    const int c0 = sys[s].cell_off + (r - rowoff[s]) * ncx;
+    int h = hstart[c0 + ncx] - hstart[c0];
     ntask[r] = (h + 63) >> 6;
 }
 
@@ -150,27 +187,14 @@ __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
 
 template <typename T>
 struct SweepArgs {
-    const Ent<T> *E;
-    const int *cstart;
     const int *hstart;
     const int *tstart;
-    int *task_counter;   // [8] per-XCD counters + [8] done flags, zeroed before launch
-    int nc[3];
-    int nrows;
-    int lcell;
-    int dim;
-    T rc2;
+    const int *rowoff;         // [nsys + 1] first global row of each system
+    const SysDesc<T> *sys;
+    int *task_counter;         // [8] per-XCD counters, zeroed before launch
+    int nrows;                 // rows over all systems
+    int nsys;
 };
-
-// Largest k in [lo, hi) with a[k] <= v (a non-decreasing, a[lo] <= v).
-__device__ inline int last_le(const int *__restrict__ a, int lo, int hi, int v) {
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (a[mid] <= v) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 // Dynamic task fetch: blocks are dealt to XCDs round-robin (block b runs on XCD b % 8),
 // so each XCD walks its own contiguous share of the tasks first (neighbouring rows share
@@ -203,8 +227,6 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
     constexpr int B = SweepBatch<T>::value;
     typename Op::Wave ws;
     op.wave_init(sh, ws);
-    const int ncx = sa.nc[0], ncy = sa.nc[1];
-    const int L = sa.lcell;
     const int ntasks = sa.tstart[sa.nrows];
     int home_xcd = blockIdx.x & 7, probe = 0;
     while (true) {
@@ -214,8 +236,14 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
         __syncthreads();
         if (task < 0) break;
         const int r = last_le(sa.tstart, 0, sa.nrows + 1, task);
+        const int si = uniform_int(last_le(sa.rowoff, 0, sa.nsys + 1, r));
+        const SysDesc<T> &sd = sa.sys[si];
+        const int ncx = sd.box.nc[0], ncy = sd.box.nc[1];
+        const int L = sd.box.lcell;
+        const T rc2 = sd.box.rc2;
+        const int lr = r - sa.rowoff[si];
         const int chunk = task - sa.tstart[r];
-        const int rowc0 = r * ncx;
+        const int rowc0 = sd.cell_off + lr * ncx;
         const int hb = sa.hstart[rowc0] + chunk * 64;
         const int he = sa.hstart[rowc0 + ncx];
         const int h = hb + lane;
@@ -224,10 +252,9 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
         const int pos = cs[c] + ((valid ? h : hb) - sa.hstart[c]);
         const Ent<T> p = E[pos];
         const int cx = c - rowc0;
-        const int cy = r % ncy, cz = r / ncy;
+        const int cy = lr % ncy, cz = lr / ncy;
         const int nvalid = min(64, he - hb);
         const int cxmin = uniform_int(cx);
-        const int cxmax = __shfl(cx, nvalid - 1, 64);
         const uint32_t wp = ent_w(p);
         const uint32_t ip = wp & IDX_MASK;
         const bool preal = !(wp & GHOST_BIT);
@@ -236,7 +263,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
         const int dzr = (D == 3) ? L : 0;
         for (int dz = -dzr; dz <= dzr; ++dz) {
             for (int dy = -L; dy <= L; ++dy) {
-                const int base = ((cz + dz) * ncy + (cy + dy)) * ncx;
+                const int base = sd.cell_off + ((cz + dz) * ncy + (cy + dy)) * ncx;
                 int xlo = -L;
                 const bool selfrow = (dz == 0 && dy == 0);
                 if (MODE == MODE_A_ORTHO) xlo = (dy > 0 || (dy == 0 && dz > 0)) ? 0 : 1;
@@ -246,7 +273,6 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                 if (!valid) { lo = 0; hi = 0; }
                 int J0 = uniform_int(cs[base + cxmin + (MODE == MODE_A_ORTHO && selfrow ? 0 : xlo)]);
                 int J1 = uniform_int(__shfl(cs[base + cx + L + 1], nvalid - 1, 64));
-                (void)cxmax;
                 // Neighbour entries are wave-uniform: they are read with scalar loads in
                 // batches of B, the next batch issued before the current one is used (E is
                 // padded by 2*B entries, so the look-ahead never leaves the allocation).
@@ -265,7 +291,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                             const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
                             T d2 = dx * dx + dy2 * dy2;
                             if (D == 3) d2 = d2 + dz2 * dz2;
-                            bool ok = d2 <= sa.rc2;
+                            bool ok = d2 <= rc2;
                             if (MODE == MODE_A_ORTHO) {
                                 ok = ok && (j >= lo) && (j < hi) && (preal || !(wq & GHOST_BIT));
                             } else if (MODE == MODE_A_TRIC) {
@@ -281,7 +307,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                 }
             }
         }
-        op.lane_end(sh, ws, ls, p, valid);
+        op.lane_end(sh, ws, ls, p, valid, sd, si);
     }
     op.wave_end(sh, ws);
 }
@@ -413,7 +439,9 @@ struct OpNeighborList {
             l.n = 0;
         }
     }
-    __device__ void lane_end(Shared &, Wave &w, Lane &l, const Ent<T> &, bool) const { w.staged = l.n; }
+    __device__ void lane_end(Shared &, Wave &w, Lane &l, const Ent<T> &, bool, const SysDesc<T> &, int) const {
+        w.staged = l.n;
+    }
     __device__ void wave_end(Shared &sh, Wave &w) const {
         if (w.staged > 0) flush(sh, w, w.staged);
         if (w.cur >= 0) retire(w, w.cur_used);
@@ -497,10 +525,8 @@ struct OpSoftCore {
     };
     SoftParams<T> prm;
     T *forces;            // n*D, input frame
-    double *energy;       // global accumulator
+    double *energy;       // per-system accumulators
     double *deps, *dsig;  // per-type adjoint accumulators (ADJ)
-    T invrot[9];
-    int tric;
     int dim;
 
     struct Wave {};
@@ -519,7 +545,7 @@ struct OpSoftCore {
         int t = 0;
         if (prm.ntypes) t = l.ti * 8 + (int)((ent_w(q) >> TYPE_SHIFT) & 7u);
         const T e = prm.eps[t];
-        const T inv = T(1) / (d2 + T(1));
+        const T inv = fast_rcp(d2 + T(1));
         const T s = prm.sig2[t] * inv;
         const T s3 = s * s * s;
         const T s6 = s3 * s3;
@@ -536,15 +562,17 @@ struct OpSoftCore {
             l.asig += e * w / prm.sig[t];
         }
     }
-    __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &p, bool valid) const {
+    __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &p, bool valid, const SysDesc<T> &sd,
+                             int si) const {
         double ev = valid ? 0.5 * (double)l.e : 0.0;
         ev = wave_sum(ev);
-        if (threadIdx.x == 0) atomicAdd(energy, ev);
+        if (threadIdx.x == 0) atomicAdd(&energy[si], ev);
         if (valid) {
             const uint32_t i = ent_w(p) & IDX_MASK;
             if (FORCES) {
                 T f[3] = {l.fx, l.fy, l.fz};
-                if (tric) {
+                if (sd.box.celltype == 1) {
+                    const T *invrot = sd.invrot;
                     T g[3];
                     for (int r = 0; r < 3; ++r) g[r] = (invrot[r] * f[0] + invrot[r + 3] * f[1]) + invrot[r + 6] * f[2];
                     for (int r = 0; r < 3; ++r) f[r] = g[r];
@@ -623,7 +651,7 @@ struct OpHistogram {
             if (d > T(0)) atomicAdd(&sh.vs[b], (double)(t / d));
         }
     }
-    __device__ void lane_end(Shared &, Wave &, Lane &, const Ent<T> &, bool) const {}
+    __device__ void lane_end(Shared &, Wave &, Lane &, const Ent<T> &, bool, const SysDesc<T> &, int) const {}
     __device__ void wave_end(Shared &sh, Wave &) const {
         __syncthreads();
         for (int k = threadIdx.x; k < nbins; k += 64) {
@@ -666,7 +694,7 @@ struct OpMinDist {
         int a = i < j ? i : j, b = i < j ? j : i;
         if (better(d2, a, b, l.d2, l.a, l.b)) { l.d2 = d2; l.a = a; l.b = b; }
     }
-    __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &, bool) const {
+    __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &, bool, const SysDesc<T> &, int) const {
         T d2 = l.d2;
         int a = l.a, b = l.b;
         for (int off = 32; off > 0; off >>= 1) {

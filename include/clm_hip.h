@@ -87,9 +87,21 @@ const char *clm_version(void);
  * (Box.jl:374-377; src/internals/ParticleSystem.jl:158-164). */
 clm_status clm_set_box(clm_ctx *ctx, const double *cell, int celltype, double cutoff, int lcell);
 clm_status clm_get_box(clm_ctx *ctx, clm_box_info *out);
+/* Box of system k of a batch (k = 0 for a single system). */
+clm_status clm_get_system_box(clm_ctx *ctx, int k, clm_box_info *out);
 /* Host-only box construction (no device needed): the same arithmetic as clm_set_box. */
 clm_status clm_box_compute(int dtype, int dim, const double *cell, int celltype, double cutoff, int lcell,
                            const double *origin, clm_box_info *out);
+
+/* A batch of nsys independent periodic systems in one context (no reference
+ * counterpart: the reference handles one system per ParticleSystem).  System k has
+ * n[k] particles, unit cell cells[k*dim*dim ...] (column-major), cutoff cutoffs[k] and
+ * lcell lcells[k]; all systems share the cell type.  Positions are then given as the
+ * concatenation of all systems (system 0 first); energies are reported per system by
+ * clm_energy_forces_batch and clm_graph_launch.  Replaces the per-system loop over
+ * ParticleSystem + pairwise! (src/API/ParticleSystem.jl:142-174, src/API/pairwise.jl:48-63). */
+clm_status clm_set_systems(clm_ctx *ctx, int nsys, const int64_t *n, const double *cells, int celltype,
+                           const double *cutoffs, const int *lcells);
 
 /* ---- particles ---------------------------------------------------------- */
 
@@ -126,6 +138,10 @@ clm_status clm_neighborlist(clm_ctx *ctx, int32_t *i, int32_t *j, void *d, int64
 clm_status clm_energy_forces(clm_ctx *ctx, const clm_softcore *p, double *energy, void *forces,
                              int on_device, int64_t *npairs);
 
+/* Soft-core energy per system of a batch (energies[nsys]) and forces of all particles. */
+clm_status clm_energy_forces_batch(clm_ctx *ctx, const clm_softcore *p, double *energies, void *forces,
+                                   int on_device);
+
 /* Energy and its adjoint in one sweep: dE/dx (n*dim, context dtype), dE/deps and
  * dE/dsigma per type (double, p->ntypes entries, or 1 when untyped).  Positions
  * are re-read, no pair data is stored.  No reference counterpart (the reference
@@ -142,6 +158,22 @@ clm_status clm_histogram(clm_ctx *ctx, int nbins, double width, int64_t *counts,
 /* Minimum distance over pairs within the cutoff (test/examples/nearest_neighbor.jl:9-16):
  * lexicographic minimum of (d, min(i,j), max(i,j)); i = j = -1, d = +inf if none. */
 clm_status clm_min_distance(clm_ctx *ctx, int64_t *i, int64_t *j, double *d);
+
+/* ---- captured pipeline ----------------------------------------------------- */
+
+/* Capture UpdateCellList! + energy/forces of the context (single system or batch) into
+ * a HIP graph; forces: device array (n*dim) or NULL for a context-owned one.  The cell
+ * list must have been built once (it sizes the entry buffer with 1/8 headroom).  No
+ * reference counterpart. */
+clm_status clm_graph_capture_energy_forces(clm_ctx *ctx, const clm_softcore *p, void *forces);
+/* Replay the captured graph on the current positions; energies[nsys] (host, may be NULL).
+ * CLM_ERR_CAPACITY when the entries outgrew the captured capacity (capture again). */
+clm_status clm_graph_launch(clm_ctx *ctx, double *energies);
+
+/* Sweep launch shape: one-wave workgroups per CU of the persistent sweep (0 = as many
+ * as fit).  Set by the autotuner; no reference counterpart (nbatches plays this role,
+ * src/internals/CellLists.jl:178-273). */
+clm_status clm_set_tuning(clm_ctx *ctx, int waves_per_cu);
 
 /* ---- measurement ----------------------------------------------------------- */
 

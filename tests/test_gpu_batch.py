@@ -1,0 +1,97 @@
+"""Batched systems (one context, one persistent sweep launch) and the captured HIP
+graph, checked system by system against the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_batch_data(k, seed, tric=False):
+    rng = np.random.default_rng(seed)
+    xs, cells, cuts, lcs = [], [], [], []
+    for s in range(k):
+        n = int(rng.integers(300, 3000))
+        rc = float(rng.uniform(4.0, 6.0))
+        L = max((n / 0.1) ** (1 / 3), 2.2 * rc)
+        if tric:
+            M = np.array([[L, 0.2 * L, 0.1 * L], [0.0, L, 0.15 * L], [0.0, 0.0, L]])
+            x = (M @ rng.random((3, n))).T
+            cells.append(M)
+        else:
+            x = rng.random((n, 3)) * L
+            cells.append(np.array([L, L, L]))
+        xs.append(x)
+        cuts.append(rc)
+        lcs.append(int(rng.integers(1, 4)))
+    return xs, cells, cuts, lcs
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("tric", [False, True])
+def test_batch_matches_oracle(clm, O, dtype, tric):
+    say(f"[batch] dtype={np.dtype(dtype).name} tric={tric}")
+    xs, cells, cuts, lcs = make_batch_data(12, 31 + tric, tric)
+    xs = [x.astype(dtype) for x in xs]
+    b = clm.BatchedSystems(xs, cells, cuts, lcell=lcs)
+    e, f = b.energy_forces(0.2, 3.0)
+    fs = b.split(f)
+    tol = 1e-10 if dtype == np.float64 else 1e-4
+    for k in range(len(xs)):
+        box = O.make_box(cells[k], cuts[k], lcell=1)
+        ref = O.softcore(box, xs[k].astype(np.float64), 0.2, 3.0)
+        assert abs(e[k] - ref["energy"]) <= tol * abs(ref["energy"]), k
+        assert np.linalg.norm(fs[k] - ref["forces"]) <= tol * np.linalg.norm(ref["forces"]), k
+
+
+def test_batch_of_one_equals_single(clm):
+    say("[batch] one system")
+    rng = np.random.default_rng(8)
+    x = rng.random((4000, 3)) * 35.0
+    b = clm.BatchedSystems([x], [[35.0, 35.0, 35.0]], 7.0, lcell=2)
+    e, f = b.energy_forces()
+    s = clm.ParticleSystem(xpositions=x, unitcell=[35.0] * 3, cutoff=7.0, lcell=2)
+    r = clm.pairwise_(clm.SoftCoreEnergyForces(), s)
+    assert e[0] == pytest.approx(r.energy, rel=1e-12)
+    assert np.allclose(f, r.forces, rtol=1e-10, atol=1e-12)
+
+
+def test_graph_replays_match_eager(clm):
+    """Every replay of the captured update + energy/forces graph equals an eager
+    evaluation of the same positions (eager work and syncs between replays)."""
+    import torch
+    say("[graph] capture + 3 replays")
+    xs, cells, cuts, lcs = make_batch_data(10, 77)
+    b = clm.BatchedSystems([x.astype(np.float64) for x in xs], cells, cuts, lcell=lcs)
+    forces = torch.empty((b.n, 3), dtype=torch.float64, device="cuda:0")
+    b.capture(0.2, 3.0, forces=forces)
+    rng = np.random.default_rng(5)
+    cur = [x.copy() for x in xs]
+    for rep in range(3):
+        cur = [x + rng.normal(0, 0.05, x.shape) for x in cur]
+        flat = torch.from_numpy(np.concatenate(cur)).cuda()
+        b.set_positions(flat)
+        eg = b.replay()
+        fg = forces.cpu().numpy().copy()
+        ee, fe = b.energy_forces()
+        torch.cuda.synchronize()
+        assert np.allclose(eg, ee, rtol=1e-12, atol=0), rep
+        assert np.allclose(fg, fe, rtol=1e-10, atol=1e-12), rep
+
+
+def test_batch_workload_shape(clm, O):
+    """the 512-system generator: redraws keep every box valid; a sample of systems checked."""
+    say("[batch] 512-system generator sample")
+    w = clm.workloads.batch_uneven(k=512)
+    assert len(w["positions"]) == 512
+    assert all(s[0] > 2 * c for s, c in zip(w["sides"], w["cutoffs"]))
+    idx = [0, 100, 311]
+    b = clm.BatchedSystems([w["positions"][i] for i in idx], [w["sides"][i] for i in idx],
+                           [w["cutoffs"][i] for i in idx], lcell=1)
+    e, f = b.energy_forces()
+    for q, i in enumerate(idx):
+        ref = O.softcore(O.make_box(w["sides"][i], w["cutoffs"][i]), w["positions"][i].astype(np.float64), forces=False)
+        assert abs(e[q] - ref["energy"]) <= 1e-4 * abs(ref["energy"])
