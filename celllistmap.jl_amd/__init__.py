@@ -10,6 +10,7 @@ from .core import (  # noqa: F401
 )
 
 from .batch import BatchedSystems  # noqa: F401
-from . import autotune, workloads  # noqa: F401
+from . import autograd, autotune, workloads  # noqa: F401
+from .autograd import softcore_energy  # noqa: F401
 
 __version__ = "0.1.0"

@@ -1,0 +1,79 @@
+"""Differentiable soft-core energy: the counterpart of a ChainRulesCore.rrule for
+``energy(positions, eps, sigma)``.
+
+forward  -> E (device sweep)
+backward -> dE/dx (N, dim), dE/deps (ntypes), dE/dsigma (ntypes), scaled by dL/dE.
+
+``fused=True`` computes E and all gradients in one sweep during the forward pass
+(the adjoint kernel re-reads positions; no pair data is stored).  ``fused=False``
+computes only E in the forward pass and runs the adjoint sweep in the backward pass.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .core import ParticleSystem, SoftCoreEnergy, energy_adjoint, pairwise_, update_
+
+__all__ = ["SoftCoreEnergyFunction", "softcore_energy"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _make_function():
+    torch = _torch()
+
+    class SoftCoreEnergyFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, positions, eps, sigma, system, types, fused):
+            update_(system, xpositions=positions.detach())
+            e_np = eps.detach().cpu().numpy().astype(np.float64)
+            s_np = sigma.detach().cpu().numpy().astype(np.float64)
+            t = types
+            if fused:
+                g = energy_adjoint(system, e_np, s_np, t, device_output=positions.is_cuda)
+                ctx.save_for_backward(torch.as_tensor(g.dedx, device=positions.device),
+                                      torch.as_tensor(g.deps, device=eps.device, dtype=eps.dtype),
+                                      torch.as_tensor(g.dsigma, device=sigma.device, dtype=sigma.dtype))
+                energy = g.energy
+            else:
+                r = pairwise_(SoftCoreEnergy(e_np, s_np, t), system)
+                ctx.system, ctx.params = system, (e_np, s_np, t)
+                ctx.devices = (positions.device, eps.device, sigma.device, eps.dtype, sigma.dtype, positions.is_cuda)
+                energy = r.energy
+            ctx.fused = fused
+            return torch.tensor(energy, dtype=torch.float64, device=positions.device)
+
+        @staticmethod
+        def backward(ctx, grad_e):
+            if ctx.fused:
+                dedx, deps, dsig = ctx.saved_tensors
+            else:
+                e_np, s_np, t = ctx.params
+                pdev, edev, sdev, edt, sdt, cuda = ctx.devices
+                g = energy_adjoint(ctx.system, e_np, s_np, t, device_output=cuda)
+                dedx = torch.as_tensor(g.dedx, device=pdev)
+                deps = torch.as_tensor(g.deps, device=edev, dtype=edt)
+                dsig = torch.as_tensor(g.dsigma, device=sdev, dtype=sdt)
+            ge = grad_e.to(dedx.dtype)
+            return (ge * dedx, grad_e.to(deps.dtype) * deps, grad_e.to(dsig.dtype) * dsig, None, None, None)
+
+    return SoftCoreEnergyFunction
+
+
+_FN = None
+
+
+def SoftCoreEnergyFunction():
+    global _FN
+    if _FN is None:
+        _FN = _make_function()
+    return _FN
+
+
+def softcore_energy(system: ParticleSystem, positions, eps, sigma, types=None, fused: bool = True):
+    """E(positions, eps, sigma) with autograd support (positions: torch (N, dim);
+    eps, sigma: torch (ntypes,) or scalars as 1-element tensors)."""
+    return SoftCoreEnergyFunction().apply(positions, eps, sigma, system, types, fused)

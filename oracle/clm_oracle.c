@@ -649,3 +649,87 @@ int orc_max_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------ */
+/* two-set entry points                                                 */
+/* ------------------------------------------------------------------ */
+
+#define CROSS_RUN(R, SUF, CB, CTX)                                                        \
+    do {                                                                                  \
+        Box_##SUF b; COPY_BOX_IN(in, b, R);                                               \
+        if (mode == 2) rc = naive_cross_##SUF(&b, (const R *)x, nx, (const R *)y, ny, CB, CTX); \
+        else {                                                                            \
+            CellList_##SUF cx, cy;                                                        \
+            rc = cl_build_##SUF(&b, (const R *)x, nx, &cx);                               \
+            if (!rc) rc = cl_build_##SUF(&b, (const R *)y, ny, &cy);                      \
+            if (!rc) rc = sweep_cross_##SUF(&b, &cx, &cy, mode == 0, CB, CTX);            \
+            cl_free_##SUF(&cx); cl_free_##SUF(&cy);                                       \
+        }                                                                                 \
+    } while (0)
+
+/* pairs (i in x, j in y) within the cutoff; modes as orc_pairs */
+int64_t orc_pairs_cross(const orc_box *in, const void *x, int64_t nx, const void *y, int64_t ny, int mode,
+                        int32_t *pi, int32_t *pj, double *pd2, double *pd, int64_t cap) {
+    pair_sink s = {0, cap, pi, pj, pd2, pd, {0, 0, 0}};
+    int rc = 0;
+    if (in->dtype == 1) CROSS_RUN(double, f64, sink_f64, &s);
+    else CROSS_RUN(float, f32, sink_f32, &s);
+    return rc ? -(int64_t)rc : s.n;
+}
+
+int orc_reduce_cross(const orc_box *in, const void *x, int64_t nx, const void *y, int64_t ny, int mode, double *out4) {
+    red_ctx r = {0, 0, 0, 0};
+    int rc = 0;
+    if (in->dtype == 1) CROSS_RUN(double, f64, red_f64, &r);
+    else CROSS_RUN(float, f32, red_f32, &r);
+    out4[0] = r.sd2; out4[1] = r.sd; out4[2] = r.sinv; out4[3] = (double)r.n;
+    return rc;
+}
+
+typedef struct { double d; int64_t i, j; } mdx_ctx;
+static void mdx_take(mdx_ctx *m, double d, int64_t i, int64_t j) {
+    if (d < m->d || (d == m->d && (i < m->i || (i == m->i && j < m->j)))) { m->d = d; m->i = i; m->j = j; }
+}
+static void mdx_f64(void *c, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
+    (void)xi; (void)xj; mdx_take((mdx_ctx *)c, sqrt(d2), i, j);
+}
+static void mdx_f32(void *c, int64_t i, int64_t j, const float *xi, const float *xj, float d2) {
+    (void)xi; (void)xj; mdx_take((mdx_ctx *)c, (double)sqrtf(d2), i, j);
+}
+
+/* minimum distance between the sets: lexicographic min of (d, i, j) */
+int orc_mindist_cross(const orc_box *in, const void *x, int64_t nx, const void *y, int64_t ny, int mode,
+                      int64_t *oi, int64_t *oj, double *od) {
+    mdx_ctx m = {INFINITY, -1, -1};
+    int rc = 0;
+    if (in->dtype == 1) CROSS_RUN(double, f64, mdx_f64, &m);
+    else CROSS_RUN(float, f32, mdx_f32, &m);
+    *oi = m.i; *oj = m.j; *od = m.d;
+    return rc;
+}
+
+/* limits(x, y) + sides (CellOperations.jl:312-324, Box.jl:38) */
+int orc_nonperiodic_sides_cross(int dtype, int dim, const void *x, int64_t nx, const void *y, int64_t ny,
+                                double cutoff, double *sides, double *origin) {
+    for (int k = 0; k < dim; ++k) { sides[k] = 0; origin[k] = 0; }
+    if (nx + ny == 0) return 0;
+    if (dtype == 1) {
+        const double *xd = (const double *)x, *yd = (const double *)y;
+        double mn[3], mxv[3];
+        for (int k = 0; k < dim; ++k) { mn[k] = INFINITY; mxv[k] = -INFINITY; }
+        for (int64_t i = 0; i < nx; ++i) for (int k = 0; k < dim; ++k) { double v = xd[i*dim+k]; if (v < mn[k]) mn[k] = v; if (v > mxv[k]) mxv[k] = v; }
+        for (int64_t i = 0; i < ny; ++i) for (int k = 0; k < dim; ++k) { double v = yd[i*dim+k]; if (v < mn[k]) mn[k] = v; if (v > mxv[k]) mxv[k] = v; }
+        double pad = 210 * cutoff / 100;
+        for (int k = 0; k < dim; ++k) { sides[k] = (mxv[k] - mn[k]) + pad; origin[k] = mn[k]; }
+    } else {
+        const float *xf = (const float *)x, *yf = (const float *)y;
+        float mn[3], mxv[3];
+        for (int k = 0; k < dim; ++k) { mn[k] = INFINITY; mxv[k] = -INFINITY; }
+        for (int64_t i = 0; i < nx; ++i) for (int k = 0; k < dim; ++k) { float v = xf[i*dim+k]; if (v < mn[k]) mn[k] = v; if (v > mxv[k]) mxv[k] = v; }
+        for (int64_t i = 0; i < ny; ++i) for (int k = 0; k < dim; ++k) { float v = yf[i*dim+k]; if (v < mn[k]) mn[k] = v; if (v > mxv[k]) mxv[k] = v; }
+        float c = (float)cutoff;
+        float pad = (210 * c) / 100;
+        for (int k = 0; k < dim; ++k) { sides[k] = (double)((mxv[k] - mn[k]) + pad); origin[k] = mn[k]; }
+    }
+    return 0;
+}

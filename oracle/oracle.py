@@ -241,3 +241,71 @@ def bench_sweep(box: Box, x, stride=1, nthreads=0):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+# ---------------------------------------------------------------------------
+# two sets (cross.jl): pairs (i in x, j in y)
+# ---------------------------------------------------------------------------
+
+def _cross_lib():
+    L = lib()
+    if not getattr(L, "_cross_ready", False):
+        P, I64, D, I = C.c_void_p, C.c_int64, C.c_double, C.c_int
+        L.orc_pairs_cross.argtypes = [C.POINTER(Box), P, I64, P, I64, I, P, P, P, P, I64]
+        L.orc_pairs_cross.restype = I64
+        L.orc_reduce_cross.argtypes = [C.POINTER(Box), P, I64, P, I64, I, P]
+        L.orc_mindist_cross.argtypes = [C.POINTER(Box), P, I64, P, I64, I, P, P, P]
+        L.orc_nonperiodic_sides_cross.argtypes = [I, I, P, I64, P, I64, D, P, P]
+        L._cross_ready = True
+    return L
+
+
+def make_nonperiodic_box_cross(x, y, cutoff, lcell=1, dtype=np.float64):
+    """Box(limits(x, y), cutoff) (Box.jl:374-377, CellOperations.jl:312-324)."""
+    x = np.ascontiguousarray(x, dtype=_np_dtype(dtype))
+    y = np.ascontiguousarray(y, dtype=_np_dtype(dtype))
+    dim = x.shape[1]
+    sides = np.zeros(3)
+    origin = np.zeros(3)
+    rc = _cross_lib().orc_nonperiodic_sides_cross(_code(dtype), dim, _ptr(x), x.shape[0], _ptr(y), y.shape[0],
+                                                  float(cutoff), _ptr(sides), _ptr(origin))
+    if rc:
+        raise OracleError(rc)
+    return make_box(sides[:dim], cutoff, lcell, dtype, celltype=NONPERIODIC, origin=origin[:dim])
+
+
+def pairs_cross(box: Box, x, y, mode=0):
+    x = _x(box, x)
+    y = _x(box, y)
+    cap = max(1024, (x.shape[0] + y.shape[0]) * 64)
+    while True:
+        i = np.empty(cap, np.int32); j = np.empty(cap, np.int32)
+        d2 = np.empty(cap, np.float64); d = np.empty(cap, np.float64)
+        n = _cross_lib().orc_pairs_cross(C.byref(box), _ptr(x), x.shape[0], _ptr(y), y.shape[0], mode, _ptr(i),
+                                         _ptr(j), _ptr(d2), _ptr(d), cap)
+        if n < 0:
+            raise OracleError(-n)
+        if n <= cap:
+            return i[:n], j[:n], d2[:n], d[:n]
+        cap = int(n)
+
+
+def reduce_cross(box: Box, x, y, mode=0):
+    x = _x(box, x)
+    y = _x(box, y)
+    out = np.zeros(4)
+    rc = _cross_lib().orc_reduce_cross(C.byref(box), _ptr(x), x.shape[0], _ptr(y), y.shape[0], mode, _ptr(out))
+    if rc:
+        raise OracleError(rc)
+    return dict(sum_d2=out[0], sum_d=out[1], sum_invd=out[2], n=int(out[3]))
+
+
+def mindist_cross(box: Box, x, y, mode=0):
+    x = _x(box, x)
+    y = _x(box, y)
+    oi = np.zeros(1, np.int64); oj = np.zeros(1, np.int64); od = np.zeros(1)
+    rc = _cross_lib().orc_mindist_cross(C.byref(box), _ptr(x), x.shape[0], _ptr(y), y.shape[0], mode, _ptr(oi),
+                                        _ptr(oj), _ptr(od))
+    if rc:
+        raise OracleError(rc)
+    return int(oi[0]), int(oj[0]), float(od[0])

@@ -685,4 +685,132 @@ static int DEF(naive)(const DEF(Box) *b, const REAL *x, int64_t n, DEF(pair_cb) 
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* two sets: cross.jl:8-129 (ref list x, target list y, same box)       */
+/* ------------------------------------------------------------------ */
+
+/* inner_loop! for a CellListPair (cross.jl:81-103): for each ref cell with real particles,
+ * the target cell of the same index with _current_cell_interactions! (cross.jl:111-129:
+ * i real, every j), then the full stencil (neighbor_cells, Box.jl:468-474) with
+ * _vicinal_cell_interactions! without the index skip (vicinal_cells.jl:4-75; orthorhombic:
+ * skip only ghost-ghost pairs; triclinic: i real). */
+static int DEF(sweep_cross)(const DEF(Box) *b, const DEF(CellList) *cx, const DEF(CellList) *cy, int faithful,
+                            DEF(pair_cb) cb, void *ctx) {
+    int D = b->dim;
+    int L = b->lcell;
+    int tric = (b->celltype == ORC_TRICLINIC);
+    int nonper = (b->celltype == ORC_NONPERIODIC);
+    REAL rc = b->cutoff, rc2 = b->cutoff_sqr;
+    int64_t maxn = 0;
+    for (int64_t c = 0; c < cy->ncells; ++c) {
+        int64_t k = cy->cell_start[c + 1] - cy->cell_start[c];
+        if (k > maxn) maxn = k;
+    }
+    REAL *px = (REAL *)malloc(sizeof(REAL) * (maxn + 1));
+    int64_t *pi = (int64_t *)malloc(sizeof(int64_t) * (maxn + 1));
+    if (!px || !pi) { free(px); free(pi); return ORC_ERR_NOMEM; }
+    int w = 2 * L + 1;
+    int ntot = (D == 3) ? w * w * w : w * w;
+    for (int64_t h = 0; h < cx->n_home; ++h) {
+        int64_t li = cx->home[h];
+        int64_t s0 = cx->cell_start[li], s1 = cx->cell_start[li + 1];
+        /* same-index target cell: every pair with a real i */
+        for (int64_t a = s0; a < s1; ++a) {
+            const DEF(Entry) *p = &cx->e[a];
+            if (!p->real) continue;
+            for (int64_t c = cy->cell_start[li]; c < cy->cell_start[li + 1]; ++c) {
+                const DEF(Entry) *q = &cy->e[c];
+                REAL d2 = DEF(d2of)(p->x, q->x, D);
+                if (d2 <= rc2) cb(ctx, p->index, q->index, p->x, q->x, d2);
+            }
+        }
+        long ci[3];
+        DEF(cartesian)(b, li, ci);
+        REAL cen_i[3] = {0, 0, 0};
+        DEF(cell_center)(b, ci, cen_i);
+        for (int t = 0; t < ntot; ++t) {
+            long off[3] = {t % w - L, (t / w) % w - L, (D == 3) ? (t / (w * w)) - L : 0};
+            if (off[0] == 0 && off[1] == 0 && off[2] == 0) continue;
+            long cj[3] = {ci[0] + off[0], ci[1] + off[1], ci[2] + off[2]};
+            int64_t lj = DEF(lin)(b, cj);
+            int64_t t0 = cy->cell_start[lj], t1 = cy->cell_start[lj + 1];
+            if (t1 == t0) continue;
+            int64_t np = 0;
+            REAL u[3] = {0, 0, 0};
+            double margin = 0;
+            if (faithful) {
+                REAL cen_j[3] = {0, 0, 0}, dc[3] = {0, 0, 0};
+                DEF(cell_center)(b, cj, cen_j);
+                for (int k = 0; k < D; ++k) dc[k] = cen_j[k] - cen_i[k];
+                REAL dcn = RSQRT(DEF(sumsq)(dc, D));
+                for (int k = 0; k < D; ++k) u[k] = dc[k] / dcn;
+                if (L == 1) margin = (double)(rc + dcn / 2);
+                else margin = (double)rc * (1.0 + sqrt((double)D) / 2.0);
+                for (int64_t c = t0; c < t1; ++c) {
+                    REAL d[3] = {0, 0, 0};
+                    for (int k = 0; k < D; ++k) d[k] = cy->e[c].x[k] - cen_i[k];
+                    REAL xp = DEF(dot)(d, u, D);
+                    if ((double)RFABS(xp) <= margin) { px[np] = xp; pi[np] = c; np++; }
+                }
+            } else {
+                for (int64_t c = t0; c < t1; ++c) { pi[np] = c; np++; }
+            }
+            for (int64_t a = s0; a < s1; ++a) {
+                const DEF(Entry) *p = &cx->e[a];
+                if (tric && !p->real) continue;
+                REAL xpi = 0;
+                if (faithful) {
+                    REAL d[3] = {0, 0, 0};
+                    for (int k = 0; k < D; ++k) d[k] = p->x[k] - cen_i[k];
+                    xpi = DEF(dot)(d, u, D);
+                }
+                for (int64_t m = 0; m < np; ++m) {
+                    if (faithful && !(RFABS(px[m] - xpi) <= rc)) continue;
+                    const DEF(Entry) *q = &cy->e[pi[m]];
+                    if (!tric && !nonper && !(p->real | q->real)) continue;
+                    REAL d2 = DEF(d2of)(p->x, q->x, D);
+                    if (d2 <= rc2) cb(ctx, p->index, q->index, p->x, q->x, d2);
+                }
+            }
+        }
+    }
+    free(px); free(pi);
+    return 0;
+}
+
+/* map_naive! for two sets (test/modules/Testing.jl:91-104) */
+static int DEF(naive_cross)(const DEF(Box) *b, const REAL *x, int64_t nx, const REAL *y, int64_t ny,
+                            DEF(pair_cb) cb, void *ctx) {
+    int D = b->dim;
+    REAL *fy = (REAL *)malloc(sizeof(REAL) * 3 * (ny + 1));
+    if (!fy) return ORC_ERR_NOMEM;
+    for (int64_t j = 0; j < ny; ++j) {
+        REAL yin[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) yin[k] = y[j * D + k];
+        DEF(wrap_fraction)(b->input, yin, &fy[3 * j], D);
+    }
+    for (int64_t i = 0; i < nx; ++i) {
+        const REAL *xi = &x[i * D];
+        REAL xin[3] = {0, 0, 0}, fi[3];
+        for (int k = 0; k < D; ++k) xin[k] = xi[k];
+        DEF(wrap_fraction)(b->input, xin, fi, D);
+        for (int64_t j = 0; j < ny; ++j) {
+            const REAL *fj = &fy[3 * j];
+            REAL dfr[3], yj[3], tmp[3];
+            for (int k = 0; k < D; ++k) {
+                REAL v = DEF(jmod1)(fj[k] - fi[k]);
+                if (v >= (REAL)0.5) v = v - 1;
+                REAL xw = v + fi[k];
+                dfr[k] = xw - fi[k];
+            }
+            DEF(matvec)(b->input, dfr, tmp, D);
+            for (int k = 0; k < D; ++k) yj[k] = tmp[k] + xi[k];
+            REAL d2 = DEF(d2of)(xi, yj, D);
+            if (d2 <= b->cutoff_sqr) cb(ctx, i, j, xi, yj, d2);
+        }
+    }
+    free(fy);
+    return 0;
+}
+
 #undef DEF

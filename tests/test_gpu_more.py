@@ -1,0 +1,81 @@
+"""2D systems, the autograd wrapper of the energy adjoint, and device-resident I/O."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def canon(res):
+    a, b, d = res.canonical()
+    return a * (1 << 32) + b, d
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("kind", ["ortho", "tric", "nonper"])
+@pytest.mark.parametrize("lcell", [1, 2])
+def test_2d_neighborlist(clm, O, dtype, kind, lcell):
+    say(f"[2d] {kind} {np.dtype(dtype).name} lcell={lcell}")
+    rng = np.random.default_rng(12)
+    if kind == "tric":
+        uc = np.array([[40.0, 9.0], [0.0, 35.0]])
+        x = (uc @ rng.random((2, 2500))).T
+    else:
+        uc = np.array([40.0, 35.0])
+        x = rng.random((2500, 2)) * uc
+    x = x.astype(dtype)
+    rc = 3.0
+    if kind == "nonper":
+        res = clm.neighborlist(xpositions=x, cutoff=rc, lcell=lcell)
+        box = O.make_nonperiodic_box(x, rc, lcell=lcell, dtype=dtype)
+    else:
+        res = clm.neighborlist(xpositions=x, cutoff=rc, unitcell=uc, lcell=lcell)
+        box = O.make_box(uc, rc, lcell=lcell, dtype=dtype)
+    kg, dg = canon(res)
+    i, j, d2, d = O.pairs(box, x, 1)
+    ko, order = O.canonical_pairs(i, j)
+    assert np.array_equal(kg, ko)
+    assert np.array_equal(dg.astype(np.float64), d[order])
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_autograd_energy(clm, O, fused):
+    import torch
+    say(f"[autograd] fused={fused}")
+    rng = np.random.default_rng(21)
+    L = 30.0
+    n = 2700
+    x = rng.random((n, 3)) * L
+    types = rng.integers(0, 4, n).astype(np.int32)
+    eps0 = np.array([0.1, 0.25, 0.4, 0.5])
+    sig0 = np.array([2.5, 2.9, 3.2, 3.5])
+    sys_ = clm.ParticleSystem(xpositions=x, unitcell=[L] * 3, cutoff=8.0)
+    xt = torch.tensor(x, device="cuda:0", requires_grad=True)
+    et = torch.tensor(eps0, device="cuda:0", requires_grad=True)
+    st = torch.tensor(sig0, device="cuda:0", requires_grad=True)
+    E = clm.softcore_energy(sys_, xt, et, st, types, fused=fused)
+    (2.0 * E).backward()
+    ref = O.softcore(O.make_box([L] * 3, 8.0), x, eps0, sig0, types, grads=True)
+    assert E.item() == pytest.approx(ref["energy"], rel=1e-10)
+    assert np.allclose(xt.grad.cpu().numpy(), -2.0 * ref["forces"], rtol=1e-9, atol=1e-10)
+    assert np.allclose(et.grad.cpu().numpy(), 2.0 * ref["deps"], rtol=1e-9)
+    assert np.allclose(st.grad.cpu().numpy(), 2.0 * ref["dsigma"], rtol=1e-9)
+
+
+def test_device_resident_roundtrip(clm, O):
+    """positions in HBM (torch), neighbour list written to device tensors."""
+    import torch
+    say("[device io]")
+    rng = np.random.default_rng(3)
+    x = rng.random((5000, 3)).astype(np.float32) * 40
+    xt = torch.from_numpy(x).cuda()
+    nl = clm.InPlaceNeighborList(xpositions=xt, cutoff=6.0, unitcell=[40.0] * 3, device_output=True)
+    res = clm.neighborlist_(nl)
+    assert res.i.is_cuda and res.d.is_cuda
+    kg, dg = canon(res)
+    i, j, d2, d = O.pairs(O.make_box([40.0] * 3, 6.0, dtype=np.float32), x, 1)
+    ko, order = O.canonical_pairs(i, j)
+    assert np.array_equal(kg, ko) and np.array_equal(dg.astype(np.float64), d[order])

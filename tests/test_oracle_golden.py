@@ -344,3 +344,74 @@ def test_histogram_oracle_closed_form(O):
         r -= L * np.round(r / L)
         vv[min(int(np.floor(dd / 0.5)), 15)] += np.dot(v[a] - v[c], r) / dd
     assert np.allclose(vs, vv, rtol=1e-9, atol=1e-9)
+
+
+def test_argon_cross_golden(O, argon):
+    """Two-set goldens: src/API/ParticleSystem.jl:121-139 (sum of d2, periodic) and
+    src/API/neighborlist.jl:292-312 (non-periodic cross list, 439 pairs)."""
+    b = O.make_box([21.0, 21.0, 21.0], 8.0)
+    for mode in (0, 1, 2):
+        r = O.reduce_cross(b, argon[:50], argon[50:], mode)
+        assert r["sum_d2"] == pytest.approx(21886.196785000004, rel=1e-12)
+    x32 = argon.astype(np.float32).astype(np.float64)
+    bn = O.make_nonperiodic_box_cross(x32[:50], x32[50:], 8.0)
+    i, j, d2, d = O.pairs_cross(bn, x32[:50], x32[50:], 0)
+    assert len(i) == 439
+    rows = {(a, c): e for a, c, e in zip(i.tolist(), j.tolist(), d.tolist())}
+    assert len(rows) == 439
+    assert rows[(0, 10)] == 4.08865164675529
+    assert rows[(0, 16)] == 5.939772435456664
+    assert rows[(0, 29)] == 2.4572288423012236
+    assert rows[(45, 47)] == 4.9269093987894745
+    assert rows[(45, 0)] == 7.99947286297016
+    # the cross list of (x, y) is the x-y block of the single-set list of x ++ y
+    ia, ja, _, da = O.pairs(O.make_nonperiodic_box(x32, 8.0), x32, 0)
+    block = {}
+    for a, c, e in zip(ia.tolist(), ja.tolist(), da.tolist()):
+        if (a < 50) != (c < 50):
+            xa, yc = (a, c - 50) if a < 50 else (c, a - 50)
+            block[(xa, yc)] = e
+    assert block == rows
+
+
+def test_cross_random_cells_property(O):
+    """Cross-set half of check_random_cells (test/modules/Testing.jl:162-240): the
+    cell-list cross sweep against brute force, random boxes/cutoffs/lcell."""
+    rng = np.random.default_rng(17)
+    trials = 0
+    while trials < 150:
+        dim = int(rng.integers(2, 4))
+        tric = bool(rng.integers(0, 2))
+        rc = 1 + rng.random()
+        lcell = int(rng.integers(1, 4))
+        m = (-10 + 20 * rng.random((dim, dim))) if tric else np.diag(-10 + 20 * rng.random(dim))
+        try:
+            b = O.make_box(m if tric else np.diag(m), rc, lcell=lcell)
+        except O.OracleError:
+            continue
+        if np.prod(b.info()["nc"]) > 100000:
+            continue
+        x = 10 * rng.random((int(rng.integers(10, 21)), dim)) - 50
+        y = 10 * rng.random((int(rng.integers(10, 21)), dim)) - 50
+        r0 = O.reduce_cross(b, x, y, 0)
+        if r0["n"] == 0:
+            continue
+        r1 = O.reduce_cross(b, x, y, 1)
+        r2 = O.reduce_cross(b, x, y, 2)
+        assert r0["n"] == r1["n"] == r2["n"]
+        assert r0["sum_d2"] == pytest.approx(r2["sum_d2"], rel=1e-10)
+        i0, j0, _, _ = O.pairs_cross(b, x, y, 0)
+        i2, j2, _, _ = O.pairs_cross(b, x, y, 2)
+        assert sorted(zip(i0.tolist(), j0.tolist())) == sorted(zip(i2.tolist(), j2.tolist()))
+        trials += 1
+
+
+def test_cross_mindist(O, argon):
+    """Cross minimum distance equals the brute-force minimum over x-y pairs."""
+    b = O.make_box([21.0, 21.0, 21.0], 8.0)
+    i, j, d = O.mindist_cross(b, argon[:50], argon[50:], 0)
+    ii, jj, _, dd = O.pairs_cross(b, argon[:50], argon[50:], 0)
+    k = int(np.argmin(dd))
+    assert (i, j) == (int(ii[k]), int(jj[k]))
+    assert d == dd[k]
+    assert O.mindist_cross(b, argon[:50], argon[50:], 2)[2] == pytest.approx(d, rel=1e-15)
