@@ -75,6 +75,110 @@ def cpu_baseline(O, M, x, rc, target_s=12.0):
     }
 
 
+def pair_count(clm, sys_):
+    """Exact within-cutoff pair count of a system: a one-bin distance histogram."""
+    h = clm.pairwise_(clm.DistanceHistogram(nbins=1, width=2.0 * sys_.cutoff + 1.0), sys_)
+    return int(h[0])
+
+
+def bench_batched(clm, torch, reps=5):
+    """512 independent systems (N_k log-uniform in [500, 50000], own cubic boxes at
+    density 0.1, cutoffs U[8, 14], float32): update + soft-core energy/forces of every
+    system.  Plain path: one ParticleSystem per system, called in a loop.  Batched path:
+    all systems in one context, lcell / launch shape from the autotuner, the whole
+    update + energy/forces pipeline captured in one HIP graph and replayed."""
+    dev = torch.device("cuda:0")
+    w = clm.workloads.batch_uneven(k=512)
+    xs = [torch.from_numpy(x).to(dev) for x in w["positions"]]
+    sides, cuts = w["sides"], w["cutoffs"]
+    # plain per-system loop
+    systems = [clm.ParticleSystem(xpositions=xs[k], unitcell=sides[k], cutoff=float(cuts[k]),
+                                  output=clm.SoftCoreEnergyForces(0.2, 3.0)) for k in range(len(xs))]
+    P = sum(pair_count(clm, s_) for s_ in systems)
+
+    def plain_step():
+        e = 0.0
+        for k, s_ in enumerate(systems):
+            clm.update_(s_, xpositions=xs[k])
+            e += clm.pairwise_(None, s_, device_output=True).energy
+        return e
+
+    plain_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e_plain = plain_step()
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / reps
+    del systems
+    # batched: autotuned lcell / waves per CU, one graph per step
+    flat = torch.cat(xs)
+
+    def make(lc):
+        return clm.BatchedSystems(flat, [s_ for s_ in sides], cuts, counts=w["counts"], lcell=lc)
+
+    key = clm.autotune.workload_key("batch512", np.float32, int(flat.shape[0]), 0.1, float(np.mean(cuts)))
+    tune = clm.autotune.tune_batch(make, key=key, reps=2, use_cache=False)
+    b = make(tune["lcell"])
+    b.set_tuning(tune["waves_per_cu"])
+    forces = torch.empty((b.n, 3), dtype=torch.float32, device=dev)
+    b.capture(0.2, 3.0, forces=forces)
+    b.set_positions(flat)
+    b.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.set_positions(flat)
+        eb = b.replay()
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {
+        "systems": len(xs), "particles": int(flat.shape[0]), "pairs": P,
+        "plain_loop_ms": plain_ms, "plain_pairs_per_s": P / (plain_ms / 1e3),
+        "batched_graph_ms": batch_ms, "batched_pairs_per_s": P / (batch_ms / 1e3),
+        "speedup": plain_ms / batch_ms,
+        "energy_rel_diff": abs(float(np.sum(eb)) - e_plain) / abs(e_plain),
+        "autotune": {"lcell": tune["lcell"], "waves_per_cu": tune["waves_per_cu"],
+                     "table": tune["table"]},
+    }
+
+
+def bench_adjoint(clm, torch, reps=5):
+    """Typed soft-core (N = 200,000 float32, 4 types, LB mixing, cutoff 12, density 0.1),
+    E(x, eps, sigma) and its gradients through the autograd function: the fused path
+    (one sweep gives E, dE/dx, dE/deps_t, dE/dsigma_t in the forward pass) against the
+    plain path (a forward energy sweep, then the adjoint sweep in the backward pass)."""
+    dev = torch.device("cuda:0")
+    x, sides, types, eps, sig = clm.workloads.typed()
+    xt = torch.from_numpy(x).to(dev).requires_grad_(True)
+    eps_t = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+    sig_t = torch.tensor(sig, dtype=torch.float64, requires_grad=True)
+    sys_ = clm.ParticleSystem(xpositions=xt.detach(), unitcell=sides, cutoff=12.0)
+    P = pair_count(clm, sys_)
+
+    def run(fused):
+        xt.grad = None
+        eps_t.grad = None
+        sig_t.grad = None
+        e = clm.softcore_energy(sys_, xt, eps_t, sig_t, types=types, fused=fused)
+        e.backward()
+        return float(e)
+
+    out = {"N": int(x.shape[0]), "pairs": P}
+    for fused in (True, False):
+        run(fused)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e = run(fused)
+        torch.cuda.synchronize()
+        out["fused_ms" if fused else "forward_then_adjoint_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+        out["energy_fused" if fused else "energy_plain"] = e
+    out["speedup_fused_vs_forward_then_adjoint"] = out["forward_then_adjoint_ms"] / out["fused_ms"]
+    out["fused_pairs_per_s"] = P / (out["fused_ms"] / 1e3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +188,7 @@ def main():
     ap.add_argument("--lcell", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ef-reps", type=int, default=5)
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint comparisons")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -181,6 +286,17 @@ def main():
           "valu_frac_of_157TF": P * 31 / (ef_ms / 1e3) / 1e12 / FP32_VALU_PEAK_TFLOPS,
           "energy": ef.energy}
 
+    batched = adjoint = None
+    if not args.no_extras:
+        del nl, res, sysx, disp
+        torch.cuda.empty_cache()
+        log("[bench] batched 512 systems: per-system loop vs one captured graph ...")
+        batched = bench_batched(clm, torch)
+        log(f"[bench] batched speedup {batched['speedup']:.2f}x")
+        log("[bench] fused energy+adjoint vs forward then adjoint ...")
+        adjoint = bench_adjoint(clm, torch)
+        log(f"[bench] adjoint speedup {adjoint['speedup_fused_vs_forward_then_adjoint']:.2f}x")
+
     cpu = None
     if not args.no_cpu:
         O = graft.load_oracle()
@@ -210,6 +326,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "energy_forces": ef,
+        "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
 

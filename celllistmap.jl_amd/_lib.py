@@ -20,9 +20,10 @@ CLM_F32, CLM_F64 = 0, 1
 ORTHORHOMBIC, TRICLINIC, NONPERIODIC = 0, 1, 2
 
 EXPORTS = [
-    "clm_create", "clm_destroy", "clm_last_error", "clm_set_stream", "clm_version",
+    "clm_create", "clm_destroy", "clm_last_error", "clm_set_stream", "clm_get_stream", "clm_version",
     "clm_set_box", "clm_get_box", "clm_get_system_box", "clm_box_compute", "clm_set_systems",
-    "clm_set_positions", "clm_set_types", "clm_set_velocities",
+    "clm_set_positions", "clm_set_types", "clm_set_velocities", "clm_set_positions_y", "clm_clear_positions_y",
+    "clm_set_types_y", "clm_set_velocities_y",
     "clm_update_cell_list", "clm_cell_list_info",
     "clm_neighborlist", "clm_energy_forces", "clm_energy_forces_batch", "clm_energy_adjoint", "clm_histogram",
     "clm_min_distance", "clm_graph_capture_energy_forces", "clm_graph_launch", "clm_set_tuning",
@@ -77,6 +78,7 @@ def lib():
         "clm_destroy": ([P], I),
         "clm_last_error": ([P], C.c_char_p),
         "clm_set_stream": ([P, P], I),
+        "clm_get_stream": ([P, P], I),
         "clm_version": ([], C.c_char_p),
         "clm_set_box": ([P, P, I, D, I], I),
         "clm_get_box": ([P, C.POINTER(BoxInfo)], I),
@@ -90,6 +92,10 @@ def lib():
         "clm_set_positions": ([P, P, I64, I], I),
         "clm_set_types": ([P, P, I64, I], I),
         "clm_set_velocities": ([P, P, I64, I], I),
+        "clm_set_positions_y": ([P, P, I64, I], I),
+        "clm_clear_positions_y": ([P], I),
+        "clm_set_types_y": ([P, P, I64, I], I),
+        "clm_set_velocities_y": ([P, P, I64, I], I),
         "clm_update_cell_list": ([P], I),
         "clm_cell_list_info": ([P, P], I),
         "clm_neighborlist": ([P, P, P, P, I64, C.POINTER(I64), I], I),

@@ -81,6 +81,8 @@ class BatchedSystems:
         if on_dev:
             self._ctx.use_torch_stream()
         self._ctx.check(self._ctx.lib.clm_set_positions(self._ctx.ptr, _ptr(xs), int(n), int(on_dev)), "positions")
+        if on_dev:
+            self._ctx.torch_after()
         self._xref = xs
 
     def set_tuning(self, waves_per_cu: int):

@@ -169,9 +169,27 @@ class _Context:
         _raise(self.ptr, code, what)
 
     def use_torch_stream(self):
+        """Order the library's stream after the work pending on torch's current stream
+        (which wrote the tensors the next call reads), and torch's stream after the
+        library's.  The library keeps its own (non-blocking) stream: it never launches
+        on torch's null stream, which graph capture cannot use."""
         torch = _torch()
-        s = torch.cuda.current_stream(self.device).cuda_stream
-        self.check(self.lib.clm_set_stream(self.ptr, C.c_void_p(s)), "set_stream")
+        ext = getattr(self, "_ext", None)
+        if ext is None:
+            sp = C.c_void_p()
+            self.check(self.lib.clm_get_stream(self.ptr, C.byref(sp)), "get_stream")
+            ext = torch.cuda.ExternalStream(sp.value, device=f"cuda:{self.device}")
+            self._ext = ext
+        cur = torch.cuda.current_stream(self.device)
+        ext.wait_stream(cur)
+        cur.wait_stream(ext)
+
+    def torch_after(self):
+        """Make torch's current stream wait for work the library left queued (an
+        asynchronous device-to-device copy of a tensor torch may overwrite next)."""
+        ext = getattr(self, "_ext", None)
+        if ext is not None:
+            _torch().cuda.current_stream(self.device).wait_stream(ext)
 
 
 def _positions(x, dtype_code: Optional[int], dim: Optional[int] = None):
@@ -253,10 +271,12 @@ class DistanceHistogram:
 
 @dataclass
 class PairwiseVelocities:
-    """Counts and mean radial velocity dot(v_i - v_j, x_i - x_j)/d per distance bin."""
+    """Counts and mean radial velocity dot(v_i - v_j, x_i - x_j)/d per distance bin
+    (``yvelocities``: the velocities of the second set of a two-set system)."""
     nbins: int = 10
     width: float = 1.0
     velocities: object = None
+    yvelocities: object = None
 
 
 @dataclass
@@ -312,24 +332,31 @@ class NeighborListResult:
         o = np.lexsort((b, a))
         return a[o], b[o], d[o]
 
+    def ordered(self):
+        """(i, j, d) sorted by (i, j) without swapping (two-set lists: i in x, j in y)."""
+        i = np.asarray(self.i.cpu() if _is_torch(self.i) else self.i).astype(np.int64)
+        j = np.asarray(self.j.cpu() if _is_torch(self.j) else self.j).astype(np.int64)
+        d = np.asarray(self.d.cpu() if _is_torch(self.d) else self.d)
+        o = np.lexsort((j, i))
+        return i[o], j[o], d[o]
+
 
 # ---------------------------------------------------------------------------
 # ParticleSystem
 # ---------------------------------------------------------------------------
 
 class ParticleSystem:
-    """ParticleSystem(; xpositions, unitcell, cutoff, output, output_name, lcell)
+    """ParticleSystem(; xpositions, ypositions, unitcell, cutoff, output, output_name, lcell)
     (src/API/ParticleSystem.jl:142-202).  ``output`` is a device operator; the
     result of the last ``pairwise_`` is available as ``sys.output`` (and under
-    ``output_name``)."""
+    ``output_name``).  With ``ypositions`` the system holds two sets and the
+    operators visit the x-y pairs (i indexing x, j indexing y)."""
 
     def __init__(self, xpositions=None, *, positions=None, ypositions=None, unitcell=None, cutoff,
                  output=None, output_name: str = "output", parallel: bool = True, nbatches=(0, 0),
                  lcell: int = 1, validate_coordinates=True, dtype=None, device: int = 0):
         if (positions is None) == (xpositions is None):
             raise ArgumentError("Either `positions` OR `xpositions` must be defined.")
-        if ypositions is not None:
-            raise NotImplementedError("two-set (xpositions, ypositions) systems are not supported by the device engine yet")
         x = xpositions if xpositions is not None else positions
         code = None if dtype is None else _dtype_code(dtype)
         xs, on_dev, n, dim, code = _positions(x, code, np.asarray(unitcell).shape[0] if unitcell is not None else None)
@@ -350,8 +377,14 @@ class ParticleSystem:
         self._unitcell = None if unitcell is None else np.asarray(unitcell, dtype=np.float64)
         self._cutoff = float(cutoff)
         self._n = n
+        self._ny = None
         self._apply_box()
         self._set_positions(xs, on_dev, n)
+        if ypositions is not None:
+            ys, y_dev, ny, ydim, _ = _positions(ypositions, code, dim)
+            if ydim != dim:
+                raise DimensionMismatch("xpositions and ypositions must have the same dimension")
+            self._set_ypositions(ys, y_dev, ny)
 
     # -- properties ------------------------------------------------------
     @property
@@ -382,6 +415,15 @@ class ParticleSystem:
     def n(self):
         return self._n
 
+    @property
+    def ny(self):
+        """Number of particles of the second set (None for a one-set system)."""
+        return self._ny
+
+    @property
+    def two_set(self) -> bool:
+        return self._ny is not None
+
     def __getattr__(self, name):
         if name != "output_name" and name == self.__dict__.get("output_name"):
             return self.__dict__.get("output")
@@ -409,8 +451,19 @@ class ParticleSystem:
         if on_dev:
             self._ctx.use_torch_stream()
         self._ctx.check(self._ctx.lib.clm_set_positions(self._ctx.ptr, _ptr(xs), int(n), int(on_dev)), "positions")
+        if on_dev:
+            self._ctx.torch_after()
         self._n = n
         self._xref = xs  # keep alive until the copy is done
+
+    def _set_ypositions(self, ys, on_dev, n):
+        if on_dev:
+            self._ctx.use_torch_stream()
+        self._ctx.check(self._ctx.lib.clm_set_positions_y(self._ctx.ptr, _ptr(ys), int(n), int(on_dev)), "ypositions")
+        if on_dev:
+            self._ctx.torch_after()
+        self._ny = n
+        self._yref = ys
 
     def update_cell_list(self):
         """UpdateCellList! (CellLists.jl:727-927) on the device."""
@@ -425,7 +478,7 @@ def update_(sys, *, xpositions=None, positions=None, ypositions=None, cutoff=Non
         return sys
     if positions is not None and xpositions is not None:
         raise ArgumentError("Either `positions` OR `xpositions` must be provided, not both.")
-    if ypositions is not None:
+    if ypositions is not None and not sys.two_set:
         raise ArgumentError("ypositions can only be set for a two-set particle system")
     x = xpositions if xpositions is not None else positions
     if x is not None:
@@ -433,6 +486,11 @@ def update_(sys, *, xpositions=None, positions=None, ypositions=None, cutoff=Non
         if dim != sys.dim:
             raise DimensionMismatch(f"\n Positions have dimension {dim}, but the unit cell has dimension {sys.dim}.\n")
         sys._set_positions(xs, on_dev, n)
+    if ypositions is not None:
+        ys, y_dev, ny, ydim, _ = _positions(ypositions, sys._ctx.dtype_code, sys.dim)
+        if ydim != sys.dim:
+            raise DimensionMismatch(f"\n Positions have dimension {ydim}, but the unit cell has dimension {sys.dim}.\n")
+        sys._set_ypositions(ys, y_dev, ny)
     if cutoff is not None:
         sys._cutoff = float(cutoff)
     if unitcell is not None:
@@ -478,8 +536,9 @@ def pairwise_(op, sys: ParticleSystem, *, reset: bool = True, show_progress: boo
         e = C.c_double(0.0)
         npairs = C.c_int64(0)
         f = None
-        if op.forces:
-            f = _out_array(device_output, (sys.n, sys.dim), sys.dtype, ctx.device)
+        if op.forces:  # two sets: the rows of x, then those of y
+            rows = sys.n + (sys.ny if sys.two_set else 0)
+            f = _out_array(device_output, (rows, sys.dim), sys.dtype, ctx.device)
         ctx.check(L.clm_energy_forces(ctx.ptr, C.byref(p), C.byref(e), _ptr(f), int(device_output), C.byref(npairs)),
                   "pairwise!")
         res = EnergyForces(e.value, f)
@@ -503,6 +562,18 @@ def pairwise_(op, sys: ParticleSystem, *, reset: bool = True, show_progress: boo
         if on_dev:
             ctx.use_torch_stream()
         ctx.check(L.clm_set_velocities(ctx.ptr, _ptr(vs), int(n), int(on_dev)), "velocities")
+        if sys.two_set:
+            yv = op.yvelocities
+            if yv is None:
+                raise ArgumentError("PairwiseVelocities of a two-set system needs yvelocities")
+            ys, y_dev, ny, ydim, _ = _positions(yv, ctx.dtype_code, sys.dim)
+            if ny != sys.ny or ydim != sys.dim:
+                raise DimensionMismatch("yvelocities must be (Ny, dim)")
+            if y_dev:
+                ctx.use_torch_stream()
+            ctx.check(L.clm_set_velocities_y(ctx.ptr, _ptr(ys), int(ny), int(y_dev)), "yvelocities")
+        if on_dev or (sys.two_set and y_dev):
+            ctx.torch_after()
         counts = np.zeros(op.nbins, np.int64)
         vsum = np.zeros(op.nbins, np.float64)
         ctx.check(L.clm_histogram(ctx.ptr, int(op.nbins), float(op.width), _ptr(counts), _ptr(vsum)), "pairwise!")
@@ -552,9 +623,17 @@ def _set_types(sys, types):
     else:
         tt = np.ascontiguousarray(types, dtype=np.int32)
         on_dev = False
-    if tt.shape[0] != sys.n:
-        raise DimensionMismatch("types must have one entry per particle")
-    ctx.check(ctx.lib.clm_set_types(ctx.ptr, _ptr(tt), int(tt.shape[0]), int(on_dev)), "types")
+    nx = sys.n
+    ny = sys.ny if getattr(sys, "two_set", False) else 0
+    if tt.shape[0] != nx + ny:
+        raise DimensionMismatch("types must have one entry per particle (x then y for two sets)")
+    if on_dev:
+        ctx.use_torch_stream()
+    ctx.check(ctx.lib.clm_set_types(ctx.ptr, _ptr(tt[:nx]), int(nx), int(on_dev)), "types")
+    if ny:
+        ctx.check(ctx.lib.clm_set_types_y(ctx.ptr, _ptr(tt[nx:]), int(ny), int(on_dev)), "types")
+    if on_dev:
+        ctx.torch_after()
     return 1
 
 
@@ -591,7 +670,8 @@ def _neighborlist(sys: ParticleSystem, capacity, device_output: bool):
         vol = abs(np.linalg.det(info["input"])) if sys.celltype != _lib.NONPERIODIC else np.prod(
             np.array(info["cbmax"]) - np.array(info["cbmin"]))
         sph = (np.pi if sys.dim == 2 else 4 * np.pi / 3) * sys.cutoff ** sys.dim
-        est = sys.n * (sys.n - 1) / 2 * sph / max(vol, 1e-300)
+        npairs_all = sys.n * sys.ny if sys.two_set else sys.n * (sys.n - 1) / 2
+        est = npairs_all * sph / max(vol, 1e-300)
         capacity = int(est * 1.1) + 1024
     while True:
         if device_output:

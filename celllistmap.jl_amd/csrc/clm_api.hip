@@ -81,6 +81,13 @@ struct clm_ctx {
     DBuf sp_i, sp_j, sp_d, ch_used, ch_holes, ch_movers, ch_hstart, ch_mstart;
     uint32_t *h_flags = nullptr;
 
+    // second particle set (two-set / cross mode): a sub-context on the same stream and
+    // box holding the y positions and their cell list
+    clm_ctx *ysub = nullptr;
+    bool two_set = false;
+    clm_ctx *root = nullptr;  // set in a sub-context: the owner (profiling, box)
+    int box_epoch = 0, sub_epoch = -1;
+
     // measurement
     bool prof = false;
     std::vector<Pending> pending;
@@ -144,6 +151,7 @@ void collect_profile(clm_ctx *c) {
 template <typename F>
 hipError_t timed(clm_ctx *c, const char *name, F &&launch) {
     hipEvent_t a = nullptr, b = nullptr;
+    if (c->root) c = c->root;  // a sub-context's kernels are timed by its owner
     if (c->prof) {
         a = take_event(c);
         b = take_event(c);
@@ -167,7 +175,8 @@ hipError_t timed(clm_ctx *c, const char *name, F &&launch) {
 clm_status sync(clm_ctx *c) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
-    if (c->prof) collect_profile(c);
+    clm_ctx *r = c->root ? c->root : c;
+    if (r->prof) collect_profile(r);
     return CLM_OK;
 }
 
@@ -274,19 +283,26 @@ clm_status box_status(clm_ctx *c, int rc) {
 
 // limits(x) + _sides_from_limits (CellOperations.jl:262-297, Box.jl:38) for the
 // non-periodic box; positions are read back to the host for this O(N) reduction.
+// With two sets the limits cover both (limits(x, y), CellOperations.jl:312-324).
 template <typename T>
 clm_status nonperiodic_box(clm_ctx *c) {
     const int D = c->dim;
-    std::vector<T> x((size_t)c->n * D);
-    if (c->n > 0) {
-        HIPCHK(c, hipMemcpyAsync(x.data(), c->pos.p, x.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    clm_ctx *y = c->two_set ? c->ysub : nullptr;
+    const int64_t ny = y ? y->n : 0;
+    const int64_t ntot = c->n + ny;
+    std::vector<T> x((size_t)ntot * D);
+    if (c->n > 0) HIPCHK(c, hipMemcpyAsync(x.data(), c->pos.p, sizeof(T) * c->n * D, hipMemcpyDeviceToHost, c->stream));
+    if (ny > 0)
+        HIPCHK(c, hipMemcpyAsync(x.data() + (size_t)c->n * D, y->pos.p, sizeof(T) * ny * D, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (ntot > 0) {
         clm_status s = sync(c);
         if (s) return s;
     }
     T mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
-    if (c->n > 0) {
+    if (ntot > 0) {
         for (int k = 0; k < D; ++k) { mn[k] = INFINITY; mx[k] = -INFINITY; }
-        for (int64_t i = 0; i < c->n; ++i)
+        for (int64_t i = 0; i < ntot; ++i)
             for (int k = 0; k < D; ++k) {
                 T v = x[(size_t)i * D + k];
                 if (v != v) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN)");
@@ -423,9 +439,10 @@ clm_status update_cell_list_t(clm_ctx *c) {
     if (!c->has_pos) return fail(c, CLM_ERR_STATE, "no positions set");
     if (c->n >= (int64_t)IDX_MASK) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
     clm_status s;
-    if (c->celltype == CLM_NONPERIODIC) {
+    if (c->celltype == CLM_NONPERIODIC && !c->root) {  // a sub-context gets its box from the owner
         if ((s = nonperiodic_box<T>(c))) return s;
         c->layout_dirty = true;
+        ++c->box_epoch;
     }
     if (c->layout_dirty && (s = layout_systems<T>(c))) return s;
     if ((s = ensure_bin_buffers<T>(c))) return s;
@@ -450,17 +467,52 @@ clm_status ensure_cell_list(clm_ctx *c) {
     return c->dtype == CLM_F32 ? update_cell_list_t<float>(c) : update_cell_list_t<double>(c);
 }
 
+// Both cell lists of a two-set context on one box (CellListPair, CellLists.jl:590-605):
+// the x list first (for a non-periodic box it sizes the box from both sets), then the
+// owner's box is handed to the y sub-context.
+clm_status ensure_cross(clm_ctx *c) {
+    clm_ctx *y = c->ysub;
+    if (!c->two_set || !y) return fail(c, CLM_ERR_STATE, "no second set of positions");
+    if (c->nsys != 1) return fail(c, CLM_ERR_ARG, "two sets need a single system");
+    if (!y->has_pos) return fail(c, CLM_ERR_STATE, "no y positions set");
+    if (c->celltype == CLM_NONPERIODIC && y->dirty) c->dirty = true;
+    clm_status s = ensure_cell_list(c);
+    if (s) return s;
+    if (c->sub_epoch != c->box_epoch) {
+        y->hbf = c->hbf;
+        y->hbd = c->hbd;
+        y->nsys = 1;
+        y->celltype = c->celltype;
+        y->cutoff_in = c->cutoff_in;
+        y->lcell = c->lcell;
+        y->has_box = true;
+        y->layout_dirty = true;
+        y->dirty = true;
+        c->sub_epoch = c->box_epoch;
+    }
+    y->waves_per_cu = c->waves_per_cu;
+    y->stream = c->stream;
+    if ((s = ensure_cell_list(y))) return fail(c, s, y->err);
+    return CLM_OK;
+}
+
 int mode_a(const clm_ctx *c) { return c->celltype == CLM_TRICLINIC ? MODE_A_TRIC : MODE_A_ORTHO; }
 
 // Per-cell home counts and per-row wave tasks of a sweep mode (no host synchronisation).
+// Home tables: the two-set modes share those of the one-set modes with the same homes.
+inline int mode_table(int mode) {
+    return mode == MODE_X_ORTHO ? MODE_A_ORTHO : (mode == MODE_X_REAL ? MODE_B : mode);
+}
+
 template <typename T>
 clm_status enqueue_mode(clm_ctx *c, int mode) {
+    mode = mode_table(mode);
     const int nc1 = c->ncells + 1;
     HIPCHK(c, ensure(c->hc, sizeof(int) * nc1));
     HIPCHK(c, ensure(c->hstart[mode], sizeof(int) * nc1));
     HIPCHK(c, ensure(c->ntask, sizeof(int) * (c->nrows + 1)));
     HIPCHK(c, ensure(c->tstart[mode], sizeof(int) * (c->nrows + 1)));
-    const int all_entries = (mode == MODE_A_ORTHO) ? 1 : 0;
+    const int all_entries = mode_all_entries(mode) ? 1 : 0;
     LAUNCH(c, "k_home_counts", hipLaunchKernelGGL(k_home_counts, dim3(blocks(nc1, 256)), dim3(256), 0, c->stream, c->ncells,
                                                   all_entries, (const int *)c->cstart.p, (const int *)c->creal.p,
                                                   (int *)c->hc.p));
@@ -474,6 +526,7 @@ clm_status enqueue_mode(clm_ctx *c, int mode) {
 }
 
 clm_status prepare_mode(clm_ctx *c, int mode) {
+    mode = mode_table(mode);
     if (c->mode_valid[mode]) return CLM_OK;
     clm_status s = c->dtype == CLM_F32 ? enqueue_mode<float>(c, mode) : enqueue_mode<double>(c, mode);
     if (s) return s;
@@ -483,6 +536,7 @@ clm_status prepare_mode(clm_ctx *c, int mode) {
 
 template <typename T>
 SweepArgs<T> sweep_args(clm_ctx *c, int mode) {
+    mode = mode_table(mode);
     SweepArgs<T> sa;
     sa.hstart = (const int *)c->hstart[mode].p;
     sa.tstart = (const int *)c->tstart[mode].p;
@@ -505,8 +559,9 @@ int occupancy_grid(clm_ctx *c, KernelT kernel) {
 }
 int sweep_grid(clm_ctx *c) { return std::max(64, c->num_cus * 16); }
 
+// Sweep with homes from context c and neighbours from context nb (nb == c for one set).
 template <typename T, int MODE, typename Op>
-clm_status run_sweep(clm_ctx *c, const char *name, const Op &op, int grid) {
+clm_status run_sweep_on(clm_ctx *c, clm_ctx *nb, const char *name, const Op &op, int grid) {
     HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
     clm_status s;
     if ((s = fill_int(c, c->counters.p, 16, 0))) return s;
@@ -514,25 +569,61 @@ clm_status run_sweep(clm_ctx *c, const char *name, const Op &op, int grid) {
     if (grid <= 0) grid = c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE, 2, Op>);
     const Ent<T> *E = (const Ent<T> *)c->E.p;
     const int *cs = (const int *)c->cstart.p;
+    const Ent<T> *En = (const Ent<T> *)nb->E.p;
+    const int *csn = (const int *)nb->cstart.p;
     if (c->dim == 3)
-        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, sa, op));
+        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
     else
-        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, sa, op));
+        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
     return CLM_OK;
+}
+
+template <typename T, int MODE, typename Op>
+clm_status run_sweep(clm_ctx *c, const char *name, const Op &op, int grid) {
+    return run_sweep_on<T, MODE>(c, c, name, op, grid);
+}
+
+// The sweep that visits every pair once in the reference's orientation: one set
+// (self.jl) or two (cross.jl, x homes against the y list).
+int mode_once(const clm_ctx *c) {
+    if (c->two_set) return c->celltype == CLM_TRICLINIC ? MODE_X_REAL : MODE_X_ORTHO;
+    return mode_a(c);
+}
+
+clm_status prepare_once(clm_ctx *c) {
+    clm_status s = c->two_set ? ensure_cross(c) : ensure_cell_list(c);
+    if (s) return s;
+    return prepare_mode(c, mode_once(c));
 }
 
 template <typename T, typename Op>
 clm_status run_mode_a(clm_ctx *c, const char *name, const Op &op, int grid) {
-    if (mode_a(c) == MODE_A_TRIC) return run_sweep<T, MODE_A_TRIC>(c, name, op, grid);
-    return run_sweep<T, MODE_A_ORTHO>(c, name, op, grid);
+    switch (mode_once(c)) {
+        case MODE_A_TRIC: return run_sweep<T, MODE_A_TRIC>(c, name, op, grid);
+        case MODE_X_ORTHO: return run_sweep_on<T, MODE_X_ORTHO>(c, c->ysub, name, op, grid);
+        case MODE_X_REAL: return run_sweep_on<T, MODE_X_REAL>(c, c->ysub, name, op, grid);
+        default: return run_sweep<T, MODE_A_ORTHO>(c, name, op, grid);
+    }
+}
+
+template <typename T, typename Op>
+int once_grid(clm_ctx *c) {
+    switch (mode_once(c)) {
+        case MODE_A_TRIC:
+            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 2, Op>);
+        case MODE_X_ORTHO:
+            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_X_ORTHO, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_X_ORTHO, 2, Op>);
+        case MODE_X_REAL:
+            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_X_REAL, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_X_REAL, 2, Op>);
+        default:
+            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 2, Op>);
+    }
 }
 
 template <typename T>
 clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_t cap, int64_t *npairs, int on_device) {
     clm_status s;
-    if ((s = ensure_cell_list(c))) return s;
-    const int ma = mode_a(c);
-    if ((s = prepare_mode(c, ma))) return s;
+    if ((s = prepare_once(c))) return s;
     int32_t *di = oi, *dj = oj;
     T *dd = (T *)od;
     if (!on_device) {
@@ -544,13 +635,7 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         dd = (T *)c->nl_d.p;
     }
     // persistent grid and a bound on the slots left unused in partially filled chunks
-    int grid;
-    if (c->dim == 3)
-        grid = ma == MODE_A_TRIC ? occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 3, OpNeighborList<T>>)
-                                 : occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 3, OpNeighborList<T>>);
-    else
-        grid = ma == MODE_A_TRIC ? occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 2, OpNeighborList<T>>)
-                                 : occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 2, OpNeighborList<T>>);
+    const int grid = once_grid<T, OpNeighborList<T>>(c);
     const int64_t ub_tasks = c->n_entries / 64 + c->nrows + 1;
     const int64_t active = std::min<int64_t>(grid, ub_tasks);
     const int64_t scap = 2 * (int64_t)NL_CHUNK * active + NL_CHUNK;
@@ -624,6 +709,8 @@ clm_status check_softcore(clm_ctx *c, const clm_softcore *p) {
     if (!p) return fail(c, CLM_ERR_ARG, "NULL soft-core parameters");
     if (p->ntypes < 0 || p->ntypes > 8) return fail(c, CLM_ERR_ARG, "ntypes must be in 0..8");
     if (p->ntypes > 0 && !c->has_types) return fail(c, CLM_ERR_STATE, "typed parameters need clm_set_types");
+    if (p->ntypes > 0 && c->two_set && !(c->ysub && c->ysub->has_types))
+        return fail(c, CLM_ERR_STATE, "typed parameters need clm_set_types_y for the second set");
     return CLM_OK;
 }
 
@@ -644,6 +731,18 @@ clm_status enqueue_softcore(clm_ctx *c, const clm_softcore *p, T *df) {
     op.dsig = (double *)c->acc.p + c->nsys + 8;
     op.dim = c->dim;
     const char *name = ADJ ? "k_sweep_energy_adjoint" : (FORCES ? "k_sweep_energy_forces" : "k_sweep_energy");
+    if (c->two_set) {
+        // x homes against the y list: every x-y pair once (energy, forces on x); then,
+        // for forces, y homes against the x list (forces on y, written after the x rows)
+        op.escale = 1.0;
+        if ((s = run_sweep_on<T, MODE_X_REAL>(c, c->ysub, name, op, 0))) return s;
+        if (!FORCES) return CLM_OK;
+        op.escale = 0.0;
+        op.forces = df + (size_t)c->n * c->dim;
+        if ((s = run_sweep_on<T, MODE_X_REAL>(c->ysub, c, name, op, 0))) return fail(c, s, c->ysub->err);
+        return CLM_OK;
+    }
+    op.escale = 0.5;  // MODE_B sees every pair from both sides
     return run_sweep<T, MODE_B>(c, name, op, 0);
 }
 
@@ -652,19 +751,27 @@ clm_status softcore_t(clm_ctx *c, const clm_softcore *p, double *energies, int p
                       double *dsig, int on_device) {
     clm_status s;
     if ((s = check_softcore(c, p))) return s;
-    if ((s = ensure_cell_list(c))) return s;
-    if ((s = prepare_mode(c, MODE_B))) return s;
+    if (c->two_set) {
+        if (ADJ) return fail(c, CLM_ERR_ARG, "the energy adjoint is for one set of particles");
+        if ((s = ensure_cross(c))) return s;
+        if ((s = prepare_mode(c, MODE_X_REAL))) return s;
+        if (FORCES && (s = prepare_mode(c->ysub, MODE_X_REAL))) return fail(c, s, c->ysub->err);
+    } else {
+        if ((s = ensure_cell_list(c))) return s;
+        if ((s = prepare_mode(c, MODE_B))) return s;
+    }
     const int D = c->dim;
+    const int64_t nrows_out = c->n + (c->two_set ? c->ysub->n : 0);  // y rows follow the x rows
     T *df = (T *)fout;
     if ((FORCES || ADJ) && (!on_device || !fout)) {
-        HIPCHK(c, ensure(c->outbuf, sizeof(T) * std::max<int64_t>(c->n * D, 1)));
+        HIPCHK(c, ensure(c->outbuf, sizeof(T) * std::max<int64_t>(nrows_out * D, 1)));
         df = (T *)c->outbuf.p;
     }
     if ((s = enqueue_softcore<T, FORCES, ADJ>(c, p, df))) return s;
     std::vector<double> acc(c->nsys + 16);
     HIPCHK(c, hipMemcpyAsync(acc.data(), c->acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, c->stream));
-    if ((FORCES || ADJ) && fout && !on_device && c->n > 0)
-        HIPCHK(c, hipMemcpyAsync(fout, df, sizeof(T) * c->n * D, hipMemcpyDeviceToHost, c->stream));
+    if ((FORCES || ADJ) && fout && !on_device && nrows_out > 0)
+        HIPCHK(c, hipMemcpyAsync(fout, df, sizeof(T) * nrows_out * D, hipMemcpyDeviceToHost, c->stream));
     if ((s = sync(c))) return s;
     if (energies) {
         if (per_system) {
@@ -693,6 +800,7 @@ clm_status graph_capture_t(clm_ctx *c, const clm_softcore *p, void *forces) {
     clm_status s;
     if ((s = check_softcore(c, p))) return s;
     if (c->celltype == CLM_NONPERIODIC) return fail(c, CLM_ERR_ARG, "graph capture needs a periodic box");
+    if (c->two_set) return fail(c, CLM_ERR_ARG, "graph capture is for one set of particles");
     if ((s = ensure_cell_list(c))) return s;
     // headroom for fluctuating ghost counts between replays
     const int64_t want = c->n_entries + c->n_entries / 8 + 1024;
@@ -760,9 +868,8 @@ clm_status histogram_t(clm_ctx *c, int nbins, double width, int64_t *counts, dou
     if (nbins < 1 || nbins > OpHistogram<T>::MAXB) return fail(c, CLM_ERR_ARG, "nbins must be in 1..512");
     if (!(width > 0)) return fail(c, CLM_ERR_ARG, "bin width must be positive");
     if (vsum && !c->has_vel) return fail(c, CLM_ERR_STATE, "velocities not set");
-    if ((s = ensure_cell_list(c))) return s;
-    const int ma = mode_a(c);
-    if ((s = prepare_mode(c, ma))) return s;
+    if (c->two_set && vsum && !(c->ysub && c->ysub->has_vel)) return fail(c, CLM_ERR_STATE, "y velocities not set");
+    if ((s = prepare_once(c))) return s;
     HIPCHK(c, ensure(c->hist_c, sizeof(unsigned long long) * nbins));
     HIPCHK(c, ensure(c->hist_v, sizeof(double) * nbins));
     if ((s = fill_int(c, c->hist_c.p, 2 * nbins, 0))) return s;
@@ -772,6 +879,7 @@ clm_status histogram_t(clm_ctx *c, int nbins, double width, int64_t *counts, dou
     op.nbins = nbins;
     op.width = T(width);
     op.vel = vsum ? (const T *)c->vel.p : nullptr;
+    op.vel_nb = vsum ? (const T *)(c->two_set ? c->ysub->vel.p : c->vel.p) : nullptr;
     const HostBox<T> &hb = hbox<T>(c);
     for (int k = 0; k < 9; ++k) op.rot[k] = hb.rot.a[k];
     op.tric = c->celltype == CLM_TRICLINIC;
@@ -791,9 +899,7 @@ template <typename T>
 clm_status mindist_t(clm_ctx *c, int64_t *oi, int64_t *oj, double *od) {
     clm_status s;
     if (c->nsys != 1) return fail(c, CLM_ERR_ARG, "minimum distance is per system (nsys must be 1)");
-    if ((s = ensure_cell_list(c))) return s;
-    const int ma = mode_a(c);
-    if ((s = prepare_mode(c, ma))) return s;
+    if ((s = prepare_once(c))) return s;
     const int grid = sweep_grid(c);
     HIPCHK(c, ensure(c->md_d2, sizeof(T) * grid));
     HIPCHK(c, ensure(c->md_a, sizeof(int) * grid));
@@ -802,6 +908,7 @@ clm_status mindist_t(clm_ctx *c, int64_t *oi, int64_t *oj, double *od) {
     op.wd2 = (T *)c->md_d2.p;
     op.wa = (int *)c->md_a.p;
     op.wb = (int *)c->md_b.p;
+    op.cross = c->two_set ? 1 : 0;
     if ((s = run_mode_a<T>(c, "k_sweep_mindist", op, grid))) return s;
     std::vector<T> d2(grid);
     std::vector<int> a(grid), b(grid);
@@ -860,6 +967,13 @@ clm_status clm_destroy(clm_ctx *c) {
     if (!c) return CLM_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->ysub) {
+        c->ysub->stream = nullptr;  // the owner's stream
+        c->ysub->own_stream = false;
+        c->ysub->root = nullptr;
+        clm_destroy(c->ysub);
+        c->ysub = nullptr;
+    }
     DBuf *bufs[] = {&c->pos, &c->types, &c->vel, &c->xw, &c->home_cell, &c->cnt, &c->creal, &c->cstart,
                     &c->cur_real, &c->cur_ghost, &c->E, &c->flags, &c->hc, &c->ntask, &c->hstart[0],
                     &c->hstart[1], &c->hstart[2], &c->tstart[0], &c->tstart[1], &c->tstart[2], &c->counters,
@@ -881,6 +995,12 @@ clm_status clm_destroy(clm_ctx *c) {
 
 const char *clm_last_error(const clm_ctx *c) { return c ? c->err.c_str() : "NULL context"; }
 
+clm_status clm_get_stream(clm_ctx *c, void **stream) {
+    if (!c || !stream) return CLM_ERR_ARG;
+    *stream = (void *)c->stream;
+    return CLM_OK;
+}
+
 clm_status clm_set_stream(clm_ctx *c, void *stream) {
     if (!c) return CLM_ERR_ARG;
     hipSetDevice(c->device);
@@ -895,6 +1015,7 @@ clm_status clm_set_stream(clm_ctx *c, void *stream) {
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return CLM_ERR_HIP;
         c->own_stream = true;
     }
+    if (c->ysub) c->ysub->stream = c->stream;
     return CLM_OK;
 }
 
@@ -945,6 +1066,7 @@ clm_status clm_set_box(clm_ctx *c, const double *cell, int celltype, double cuto
     c->has_box = true;
     c->dirty = true;
     c->layout_dirty = true;
+    ++c->box_epoch;
     return CLM_OK;
 }
 
@@ -982,6 +1104,7 @@ clm_status clm_set_systems(clm_ctx *c, int nsys, const int64_t *n, const double 
     c->hbf = vf;
     c->hbd = vd;
     c->nsys = nsys;
+    if (c->two_set) return fail(c, CLM_ERR_ARG, "a batch of systems holds one set of particles per system");
     c->sys_n.assign(n, n + nsys);
     c->celltype = celltype;
     c->cutoff_in = cutoffs[0];
@@ -989,6 +1112,7 @@ clm_status clm_set_systems(clm_ctx *c, int nsys, const int64_t *n, const double 
     c->has_box = true;
     c->dirty = true;
     c->layout_dirty = true;
+    ++c->box_epoch;
     return CLM_OK;
 }
 
@@ -1001,7 +1125,7 @@ clm_status clm_get_system_box(clm_ctx *c, int k, clm_box_info *out) {
     if (!c->has_box) return fail(c, CLM_ERR_STATE, "no box set");
     if (k < 0 || k >= c->nsys) return fail(c, CLM_ERR_ARG, "system index out of range");
     if (c->celltype == CLM_NONPERIODIC) {
-        clm_status s = ensure_cell_list(c);
+        clm_status s = c->two_set ? ensure_cross(c) : ensure_cell_list(c);
         if (s) return s;
     }
     if (c->dtype == CLM_F32) box_info(c->hbf[k], out);
@@ -1076,9 +1200,60 @@ clm_status clm_update_cell_list(clm_ctx *c) {
     if (!c) return CLM_ERR_ARG;
     hipSetDevice(c->device);
     c->dirty = true;
-    clm_status s = ensure_cell_list(c);
+    if (c->two_set && c->ysub) c->ysub->dirty = true;
+    clm_status s = c->two_set ? ensure_cross(c) : ensure_cell_list(c);
     if (s) return s;
     return sync(c);
+}
+
+// Second set of particles (xpositions/ypositions of ParticleSystem, ParticleSystem.jl:149-214).
+clm_status clm_set_positions_y(clm_ctx *c, const void *y, int64_t n, int on_device) {
+    if (!c) return CLM_ERR_ARG;
+    if (c->root) return fail(c, CLM_ERR_ARG, "not an owner context");
+    if (c->nsys != 1) return fail(c, CLM_ERR_ARG, "a batch of systems holds one set of particles per system");
+    hipSetDevice(c->device);
+    if (!c->ysub) {
+        clm_ctx *y2 = new clm_ctx();
+        y2->device = c->device;
+        y2->dtype = c->dtype;
+        y2->dim = c->dim;
+        y2->num_cus = c->num_cus;
+        y2->stream = c->stream;
+        y2->own_stream = false;
+        y2->root = c;
+        if (hipHostMalloc((void **)&y2->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault) != hipSuccess) {
+            delete y2;
+            return fail(c, CLM_ERR_HIP, "hipHostMalloc");
+        }
+        c->ysub = y2;
+        c->sub_epoch = -1;
+    }
+    c->ysub->stream = c->stream;
+    clm_status s = clm_set_positions(c->ysub, y, n, on_device);
+    if (s) return fail(c, s, c->ysub->err);
+    c->two_set = true;
+    if (c->celltype == CLM_NONPERIODIC) c->dirty = true;
+    return CLM_OK;
+}
+
+clm_status clm_clear_positions_y(clm_ctx *c) {
+    if (!c) return CLM_ERR_ARG;
+    c->two_set = false;
+    return CLM_OK;
+}
+
+clm_status clm_set_types_y(clm_ctx *c, const int32_t *t, int64_t n, int on_device) {
+    if (!c) return CLM_ERR_ARG;
+    if (!c->two_set || !c->ysub) return fail(c, CLM_ERR_STATE, "no second set of positions");
+    clm_status s = clm_set_types(c->ysub, t, n, on_device);
+    return s ? fail(c, s, c->ysub->err) : CLM_OK;
+}
+
+clm_status clm_set_velocities_y(clm_ctx *c, const void *v, int64_t n, int on_device) {
+    if (!c) return CLM_ERR_ARG;
+    if (!c->two_set || !c->ysub) return fail(c, CLM_ERR_STATE, "no second set of positions");
+    clm_status s = clm_set_velocities(c->ysub, v, n, on_device);
+    return s ? fail(c, s, c->ysub->err) : CLM_OK;
 }
 
 clm_status clm_cell_list_info(clm_ctx *c, int64_t *out4) {

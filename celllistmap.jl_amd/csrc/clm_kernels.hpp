@@ -20,13 +20,23 @@
 //   MODE_B        each pair from both sides (homes = real particles, full stencil,
 //                 index_i != index_j); operators accumulate only into the home
 //                 particle, so no atomics are needed for forces.
+// Two sets (cross.jl): the homes come from the cell list of one set and the
+// neighbours from the cell list of the other (same box and grid).
+//   MODE_X_ORTHO  reference orientation for orthorhombic/non-periodic boxes: homes =
+//                 all entries of cells holding a real particle, full stencil; in the
+//                 cell of the same index the home must be real, elsewhere at least one
+//                 of the two (cross.jl:81-130).
+//   MODE_X_REAL   homes = real particles, full stencil, every neighbour entry: each
+//                 pair once per real home (the triclinic cross sweep, and the force
+//                 sweeps of either set for any box).
 #pragma once
 
 #include "clm_device.hpp"
 
 namespace clm {
 
-enum { MODE_A_ORTHO = 0, MODE_A_TRIC = 1, MODE_B = 2 };
+enum { MODE_A_ORTHO = 0, MODE_A_TRIC = 1, MODE_B = 2, MODE_X_ORTHO = 3, MODE_X_REAL = 4 };
+constexpr bool mode_all_entries(int m) { return m == MODE_A_ORTHO || m == MODE_X_ORTHO; }
 enum { NONPER_FLAG = 2 };
 
 // ------------------------------------------------------------------------------
@@ -218,8 +228,11 @@ struct SweepBatch {
 };
 constexpr int SWEEP_PAD = 16;  // entries of padding after E (>= 2 * batch)
 
+// E/cs: cell-sorted entries and cell starts of the home set; En/csn: those of the
+// neighbour set (the same arrays for one set).
 template <typename T, int MODE, int D, typename Op>
 __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, const int *__restrict__ cs,
+                                              const Ent<T> *__restrict__ En, const int *__restrict__ csn,
                                               const SweepArgs<T> sa, const Op op) {
     __shared__ typename Op::Shared sh;
     __shared__ int s_task;
@@ -267,21 +280,24 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                 int xlo = -L;
                 const bool selfrow = (dz == 0 && dy == 0);
                 if (MODE == MODE_A_ORTHO) xlo = (dy > 0 || (dy == 0 && dz > 0)) ? 0 : 1;
-                int lo = cs[base + cx + xlo];
-                int hi = cs[base + cx + L + 1];
+                int lo = csn[base + cx + xlo];
+                int hi = csn[base + cx + L + 1];
                 if (MODE == MODE_A_ORTHO && selfrow) lo = pos + 1;
                 if (!valid) { lo = 0; hi = 0; }
-                int J0 = uniform_int(cs[base + cxmin + (MODE == MODE_A_ORTHO && selfrow ? 0 : xlo)]);
-                int J1 = uniform_int(__shfl(cs[base + cx + L + 1], nvalid - 1, 64));
+                // MODE_X_ORTHO: entries of the same-index cell pair only with a real home
+                int slo = 0, shi = 0;
+                if (MODE == MODE_X_ORTHO && selfrow) { slo = csn[base + cx]; shi = csn[base + cx + 1]; }
+                int J0 = uniform_int(csn[base + cxmin + (MODE == MODE_A_ORTHO && selfrow ? 0 : xlo)]);
+                int J1 = uniform_int(__shfl(csn[base + cx + L + 1], nvalid - 1, 64));
                 // Neighbour entries are wave-uniform: they are read with scalar loads in
                 // batches of B, the next batch issued before the current one is used (E is
                 // padded by 2*B entries, so the look-ahead never leaves the allocation).
                 Ent<T> cur[B], nxt[B];
 #pragma unroll
-                for (int u = 0; u < B; ++u) cur[u] = E[J0 + u];
+                for (int u = 0; u < B; ++u) cur[u] = En[J0 + u];
                 for (int j0 = J0; j0 < J1; j0 += B) {
 #pragma unroll
-                    for (int u = 0; u < B; ++u) nxt[u] = E[j0 + B + u];
+                    for (int u = 0; u < B; ++u) nxt[u] = En[j0 + B + u];
 #pragma unroll
                     for (int u = 0; u < B; ++u) {
                         const int j = j0 + u;
@@ -296,6 +312,11 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                                 ok = ok && (j >= lo) && (j < hi) && (preal || !(wq & GHOST_BIT));
                             } else if (MODE == MODE_A_TRIC) {
                                 ok = ok && (j >= lo) && (j < hi) && (ip < (wq & IDX_MASK));
+                            } else if (MODE == MODE_X_ORTHO) {
+                                const bool same = (j >= slo) && (j < shi);
+                                ok = ok && (j >= lo) && (j < hi) && (preal || (!same && !(wq & GHOST_BIT)));
+                            } else if (MODE == MODE_X_REAL) {
+                                ok = ok && (j >= lo) && (j < hi);
                             } else {
                                 ok = ok && valid && (ip != (wq & IDX_MASK));
                             }
@@ -526,6 +547,7 @@ struct OpSoftCore {
     SoftParams<T> prm;
     T *forces;            // n*D, input frame
     double *energy;       // per-system accumulators
+    double escale;        // weight of a home's pair energies: 1/2 when every pair is seen twice
     double *deps, *dsig;  // per-type adjoint accumulators (ADJ)
     int dim;
 
@@ -564,7 +586,7 @@ struct OpSoftCore {
     }
     __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &p, bool valid, const SysDesc<T> &sd,
                              int si) const {
-        double ev = valid ? 0.5 * (double)l.e : 0.0;
+        double ev = valid ? escale * (double)l.e : 0.0;
         ev = wave_sum(ev);
         if (threadIdx.x == 0) atomicAdd(&energy[si], ev);
         if (valid) {
@@ -607,7 +629,8 @@ struct OpHistogram {
     };
     int nbins;
     T width;
-    const T *vel;       // n*D input frame, may be null
+    const T *vel;       // n*D input frame, may be null (velocities of the home set)
+    const T *vel_nb;    // velocities of the neighbour set (== vel for one set)
     T rot[9];           // maps input-frame velocities to the aligned frame
     int tric;
     int dim;
@@ -645,7 +668,7 @@ struct OpHistogram {
         if (vel) {
             const uint32_t j = ent_w(q) & IDX_MASK;
             T v[3] = {T(0), T(0), T(0)}, o[3];
-            for (int k = 0; k < dim; ++k) v[k] = vel[(size_t)j * dim + k];
+            for (int k = 0; k < dim; ++k) v[k] = vel_nb[(size_t)j * dim + k];
             rotv(v, o);
             const T t = (l.vx - o[0]) * (p.x - q.x) + (l.vy - o[1]) * (p.y - q.y) + (l.vz - o[2]) * (p.z - q.z);
             if (d > T(0)) atomicAdd(&sh.vs[b], (double)(t / d));
@@ -675,6 +698,7 @@ struct OpMinDist {
     T *wd2;      // [gridDim.x]
     int *wa;     // [gridDim.x]
     int *wb;     // [gridDim.x]
+    int cross;   // two sets: keep (home, neighbour) order instead of (min, max)
 
     __host__ __device__ static bool better(T d2, int a, int b, T e2, int ea, int eb) {
         return d2 < e2 || (d2 == e2 && (a < ea || (a == ea && b < eb)));
@@ -691,7 +715,7 @@ struct OpMinDist {
     __device__ void candidate(Shared &, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         if (!ok) return;
         int i = (int)(ent_w(p) & IDX_MASK), j = (int)(ent_w(q) & IDX_MASK);
-        int a = i < j ? i : j, b = i < j ? j : i;
+        int a = (cross || i < j) ? i : j, b = (cross || i < j) ? j : i;
         if (better(d2, a, b, l.d2, l.a, l.b)) { l.d2 = d2; l.a = a; l.b = b; }
     }
     __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &, bool, const SysDesc<T> &, int) const {

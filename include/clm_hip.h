@@ -76,6 +76,8 @@ clm_status clm_destroy(clm_ctx *ctx);
 const char *clm_last_error(const clm_ctx *ctx);
 /* Launch on an existing HIP stream (hipStream_t passed as void*); NULL = the context's own. */
 clm_status clm_set_stream(clm_ctx *ctx, void *stream);
+/* The stream the context launches on (to order a caller's own device work against it). */
+clm_status clm_get_stream(clm_ctx *ctx, void **stream);
 /* Library version string. */
 const char *clm_version(void);
 
@@ -113,6 +115,19 @@ clm_status clm_set_types(clm_ctx *ctx, const int32_t *types, int64_t n, int on_d
 /* Per-particle velocities (n*dim, context dtype) for the radial-velocity histogram. */
 clm_status clm_set_velocities(clm_ctx *ctx, const void *v, int64_t n, int on_device);
 
+/* Second set of particles: ParticleSystem(xpositions=..., ypositions=...) and
+ * update!(sys; ypositions=y) (src/API/ParticleSystem.jl:142-174, src/API/updating.jl:
+ * 165-187; CellListPair, src/internals/CellLists.jl:590-605).  After this call the
+ * context is a two-set system: the pair operators visit every (i in x, j in y) pair
+ * within the cutoff once (src/internals/cross.jl:81-130), i indexing x and j y; a
+ * non-periodic box covers both sets (limits(x, y)).  Single system only. */
+clm_status clm_set_positions_y(clm_ctx *ctx, const void *y, int64_t n, int on_device);
+/* Back to one set (the y positions are kept, but no longer used). */
+clm_status clm_clear_positions_y(clm_ctx *ctx);
+/* Types / velocities of the second set (as clm_set_types / clm_set_velocities). */
+clm_status clm_set_types_y(clm_ctx *ctx, const int32_t *types, int64_t n, int on_device);
+clm_status clm_set_velocities_y(clm_ctx *ctx, const void *v, int64_t n, int on_device);
+
 /* UpdateCellList!(x, box, cl): src/internals/CellLists.jl:644-662,727-927.
  * Operators call it lazily when positions or box changed (UpdateParticleSystem!). */
 clm_status clm_update_cell_list(clm_ctx *ctx);
@@ -127,14 +142,17 @@ clm_status clm_cell_list_info(clm_ctx *ctx, int64_t *out4);
  * within the cutoff, d in the context dtype, order unspecified.  If the pair count
  * exceeds capacity, *npairs holds the required count, CLM_ERR_CAPACITY is returned
  * and the array contents are unspecified (call again with a larger capacity).
- * At most 2^31-1 pairs per call. */
+ * At most 2^31-1 pairs per call.  Two-set context: i indexes x, j indexes y
+ * (neighborlist(; xpositions, ypositions), src/API/neighborlist.jl:292-312). */
 clm_status clm_neighborlist(clm_ctx *ctx, int32_t *i, int32_t *j, void *d, int64_t capacity,
                             int64_t *npairs, int on_device);
 
 /* pairwise!(f, sys) with f = soft-core energy and forces (F_i = -dE/dx_i, n*dim,
  * context dtype, input frame).  forces may be NULL (energy only).  energy is
  * accumulated in double.  Replaces the user closure of
- * docs/src/ParticleSystem/examples.md:60-106 / test/examples/gravitational_force.jl:37-46. */
+ * docs/src/ParticleSystem/examples.md:60-106 / test/examples/gravitational_force.jl:37-46.
+ * Two-set context: the energy of the x-y pairs; forces holds (nx + ny)*dim values,
+ * the rows of x first, then those of y. */
 clm_status clm_energy_forces(clm_ctx *ctx, const clm_softcore *p, double *energy, void *forces,
                              int on_device, int64_t *npairs);
 
@@ -156,7 +174,8 @@ clm_status clm_energy_adjoint(clm_ctx *ctx, const clm_softcore *p, double *energ
 clm_status clm_histogram(clm_ctx *ctx, int nbins, double width, int64_t *counts, double *vsum);
 
 /* Minimum distance over pairs within the cutoff (test/examples/nearest_neighbor.jl:9-16):
- * lexicographic minimum of (d, min(i,j), max(i,j)); i = j = -1, d = +inf if none. */
+ * lexicographic minimum of (d, min(i,j), max(i,j)) -- of (d, i, j) with i in x and
+ * j in y for a two-set context; i = j = -1, d = +inf if none. */
 clm_status clm_min_distance(clm_ctx *ctx, int64_t *i, int64_t *j, double *d);
 
 /* ---- captured pipeline ----------------------------------------------------- */
