@@ -6,11 +6,11 @@ from ._lib import LibraryNotBuilt, build_library, lib  # noqa: F401
 from .core import (  # noqa: F401
     ArgumentError, CapacityError, DimensionMismatch, DistanceHistogram, EnergyForces, InPlaceNeighborList,
     MinimumDistance, NeighborListOp, NeighborListResult, PairwiseVelocities, ParticleSystem, SoftCoreEnergy,
-    SoftCoreEnergyForces, VelocityHistogram, box_info, EnergyGradients, energy_adjoint, neighborlist, neighborlist_, pairwise, pairwise_, update_,
+    SoftCoreEnergyForces, LennardJones, VelocityHistogram, box_info, EnergyGradients, energy_adjoint, neighborlist, neighborlist_, pairwise, pairwise_, update_,
 )
 
 from .batch import BatchedSystems  # noqa: F401
-from . import autograd, autotune, workloads  # noqa: F401
+from . import autograd, autotune, dcd, workloads  # noqa: F401
 from .autograd import softcore_energy  # noqa: F401
 
 __version__ = "0.1.0"

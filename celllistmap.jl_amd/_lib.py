@@ -44,7 +44,7 @@ class BoxInfo(C.Structure):
 
 
 class SoftCore(C.Structure):
-    _fields_ = [("ntypes", C.c_int), ("eps", C.c_double * 8), ("sigma", C.c_double * 8)]
+    _fields_ = [("ntypes", C.c_int), ("eps", C.c_double * 8), ("sigma", C.c_double * 8), ("form", C.c_int)]
 
 
 def build_library(verbose: bool = False) -> str:

@@ -236,9 +236,11 @@ class SoftCoreEnergyForces:
     sigma: object = 3.0
     types: object = None
     forces: bool = True
+    FORM = 0  # clm_potential_form
 
     def params(self) -> SoftCore:
         p = SoftCore()
+        p.form = self.FORM
         eps = np.atleast_1d(np.asarray(self.eps, np.float64))
         sig = np.atleast_1d(np.asarray(self.sigma, np.float64))
         if eps.shape != sig.shape or eps.shape[0] > 8:
@@ -254,6 +256,14 @@ class SoftCoreEnergyForces:
 @dataclass
 class SoftCoreEnergy(SoftCoreEnergyForces):
     forces: bool = False
+
+
+@dataclass
+class LennardJones(SoftCoreEnergyForces):
+    """u(d) = eps_ij [ (sigma_ij/d)^12 - 2 (sigma_ij/d)^6 ] (sigma = position of the minimum),
+    the closure of test/applications/namd/ParticleSystem_vs_NAMD.jl:13-19; forces
+    F_i = -dE/dx_i unless forces=False.  Same mixing and types as SoftCoreEnergyForces."""
+    FORM = 1
 
 
 @dataclass

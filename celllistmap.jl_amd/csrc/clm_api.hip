@@ -691,6 +691,10 @@ SoftParams<T> soft_params(const clm_softcore *p) {
     std::memset(&sp, 0, sizeof(sp));
     const int nt = p->ntypes > 0 ? p->ntypes : 1;
     sp.ntypes = p->ntypes > 0 ? p->ntypes : 0;
+    const bool lj = p->form == CLM_POT_LJ;
+    sp.a = lj ? T(0) : T(1);
+    sp.c = lj ? T(2) : T(1);
+    sp.c3 = lj ? T(6) : T(3);
     for (int a = 0; a < nt; ++a) {
         sp.eps_t[a] = T(p->eps[a]);
         for (int b = 0; b < nt; ++b) {
@@ -708,6 +712,7 @@ SoftParams<T> soft_params(const clm_softcore *p) {
 clm_status check_softcore(clm_ctx *c, const clm_softcore *p) {
     if (!p) return fail(c, CLM_ERR_ARG, "NULL soft-core parameters");
     if (p->ntypes < 0 || p->ntypes > 8) return fail(c, CLM_ERR_ARG, "ntypes must be in 0..8");
+    if (p->form != CLM_POT_SOFTCORE && p->form != CLM_POT_LJ) return fail(c, CLM_ERR_ARG, "unknown potential form");
     if (p->ntypes > 0 && !c->has_types) return fail(c, CLM_ERR_STATE, "typed parameters need clm_set_types");
     if (p->ntypes > 0 && c->two_set && !(c->ysub && c->ysub->has_types))
         return fail(c, CLM_ERR_STATE, "typed parameters need clm_set_types_y for the second set");

@@ -521,17 +521,21 @@ __global__ void k_nl_fill(long long nchunks, const int *__restrict__ used, const
     }
 }
 
-// Soft-core parameters for the device: pair tables of eps_ij and sigma_ij^2 (LB mixing).
+// Pair-potential parameters for the device: pair tables of eps_ij and sigma_ij^2 (LB
+// mixing) and the form constants of u = eps_ij [s^6 - c s^3], s = sigma_ij^2/(d2 + a):
+// soft-core a = 1, c = 1; Lennard-Jones (r_min form) a = 0, c = 2.
 template <typename T>
 struct SoftParams {
     T eps[64];
     T sig2[64];
     T sig[64];
     T eps_t[8];
+    T a, c, c3;  // c3 = 3c
     int ntypes;  // 0 = untyped (table entry 0)
 };
 
-// Energy (+ forces) from both sides: each lane owns one real particle, u/2 per visit.
+// Energy (+ forces, + parameter adjoint) of a pair potential; in mode B each lane owns
+// one real particle and takes u/2 per visit.
 template <typename T, bool FORCES, bool ADJ>
 struct OpSoftCore {
     struct Shared {
@@ -567,12 +571,12 @@ struct OpSoftCore {
         int t = 0;
         if (prm.ntypes) t = l.ti * 8 + (int)((ent_w(q) >> TYPE_SHIFT) & 7u);
         const T e = prm.eps[t];
-        const T inv = fast_rcp(d2 + T(1));
+        const T inv = fast_rcp(d2 + prm.a);
         const T s = prm.sig2[t] * inv;
         const T s3 = s * s * s;
         const T s6 = s3 * s3;
-        l.e += e * (s6 - s3);
-        const T w = T(6) * s6 - T(3) * s3;
+        l.e += e * (s6 - prm.c * s3);
+        const T w = T(6) * s6 - prm.c3 * s3;
         if (FORCES) {
             const T g = T(2) * e * w * inv;
             l.fx = fma(g, p.x - q.x, l.fx);
@@ -580,7 +584,7 @@ struct OpSoftCore {
             l.fz = fma(g, p.z - q.z, l.fz);
         }
         if (ADJ) {
-            l.aeps += (s6 - s3) * e;
+            l.aeps += (s6 - prm.c * s3) * e;
             l.asig += e * w / prm.sig[t];
         }
     }

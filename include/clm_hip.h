@@ -59,13 +59,18 @@ typedef struct {
     double origin[3];
 } clm_box_info;
 
-/* Soft-core pair potential u(d2) = eps_ij[(s)^6 - (s)^3], s = sigma_ij^2/(d2+1).
+/* Pair potential of the energy/force operators, u(d2) = eps_ij [s^6 - c s^3]:
+ *   form CLM_POT_SOFTCORE: s = sigma_ij^2/(d2+1), c = 1 (the soft-core potential);
+ *   form CLM_POT_LJ:       s = sigma_ij^2/d2,     c = 2, i.e. eps[(sigma/d)^12 - 2(sigma/d)^6],
+ *     the Lennard-Jones closure of test/applications/namd/ParticleSystem_vs_NAMD.jl:13-19.
  * ntypes = 0: a single (eps[0], sigma[0]) for every pair.  ntypes > 0: per-type
  * eps/sigma with Lorentz-Berthelot mixing, particle types from clm_set_types. */
+typedef enum { CLM_POT_SOFTCORE = 0, CLM_POT_LJ = 1 } clm_potential_form;
 typedef struct {
     int ntypes;
     double eps[8];
     double sigma[8];
+    int form;
 } clm_softcore;
 
 /* ---- context ---------------------------------------------------------- */

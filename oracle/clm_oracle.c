@@ -299,11 +299,17 @@ int64_t orc_pairs(const orc_box *in, const void *x, int64_t n, int mode,
 /* ------------------------------------------------------------------ */
 
 /*
- * u(d2) = eps_ij [ s^6 - s^3 ],  s = sigma_ij^2 / (d2 + 1)
+ * Two pair forms share this code (form 0 and form 1):
+ *   form 0, soft-core:      u = eps_ij [ s^6 -   s^3 ],  s = sigma_ij^2 / (d2 + 1)
+ *   form 1, Lennard-Jones:  u = eps_ij [ s^6 - 2 s^3 ],  s = sigma_ij^2 / d2
+ * Form 1 is the r_min form eps[(sigma/d)^12 - 2(sigma/d)^6] of the reference's NAMD
+ * comparison (test/applications/namd/ParticleSystem_vs_NAMD.jl:13-19), written with
+ * s = sigma^2/d2 instead of powers of sigma/d.  With a = (form 0 ? 1 : 0), c = (form 0 ? 1 : 2):
+ * u(d2) = eps_ij [ s^6 - c s^3 ],  s = sigma_ij^2 / (d2 + a)
  * Lorentz-Berthelot: sigma_ij = (sigma_i + sigma_j)/2, eps_ij = sqrt(eps_i eps_j).
- * du/dd2 = -eps_ij (6 s^6 - 3 s^3) / (d2 + 1);  F_i = -2 du/dd2 (x_i - x_j).
- * dE/d eps_t  += (s^6 - s^3) * eps_ij / (2 eps_t)   for t in {t_i, t_j}
- * dE/d sigma_t += eps_ij (6 s^6 - 3 s^3) / sigma_ij  for t in {t_i, t_j}  (= 1/2 * 2 eps (6s^6-3s^3)/sigma)
+ * du/dd2 = -eps_ij (6 s^6 - 3c s^3) / (d2 + a);  F_i = -2 du/dd2 (x_i - x_j).
+ * dE/d eps_t  += (s^6 - c s^3) * eps_ij / (2 eps_t)   for t in {t_i, t_j}
+ * dE/d sigma_t += eps_ij (6 s^6 - 3c s^3) / sigma_ij  for t in {t_i, t_j}  (= 1/2 * 2 eps (6s^6-3cs^3)/sigma)
  * Pair vectors are mapped back to the input frame with inv_rotation, as NeighborPair
  * does for triclinic cells (self.jl:171-178).
  */
@@ -316,6 +322,7 @@ typedef struct {
     int dim;
     const double *invrot;
     int64_t npairs;
+    double a, c;      /* form constants (see above) */
 } sc_ctx;
 
 static void sc_pair(void *c, int64_t i, int64_t j, const double *xi, const double *xj, double d2) {
@@ -323,13 +330,13 @@ static void sc_pair(void *c, int64_t i, int64_t j, const double *xi, const doubl
     int ti = s->types ? s->types[i] : 0, tj = s->types ? s->types[j] : 0;
     double e = sqrt(s->eps[ti] * s->eps[tj]);
     double sg = (s->sig[ti] + s->sig[tj]) / 2;
-    double inv = 1.0 / (d2 + 1.0);
+    double inv = 1.0 / (d2 + s->a);
     double sv = sg * sg * inv;
     double s3 = sv * sv * sv, s6 = s3 * s3;
-    s->energy += e * (s6 - s3);
+    s->energy += e * (s6 - s->c * s3);
     s->npairs++;
     if (s->forces) {
-        double g = 2.0 * e * (6.0 * s6 - 3.0 * s3) * inv;   /* F_i = g (x_i - x_j) */
+        double g = 2.0 * e * (6.0 * s6 - 3.0 * s->c * s3) * inv;   /* F_i = g (x_i - x_j) */
         double r[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
         for (int k = 0; k < s->dim; ++k) r[k] = xi[k] - xj[k];
         matvec_f64(s->invrot, r, ro, s->dim);
@@ -339,12 +346,12 @@ static void sc_pair(void *c, int64_t i, int64_t j, const double *xi, const doubl
         }
     }
     if (s->deps) {
-        double ge = s6 - s3;
+        double ge = s6 - s->c * s3;
         s->deps[ti] += ge * e / (2.0 * s->eps[ti]);
         s->deps[tj] += ge * e / (2.0 * s->eps[tj]);
     }
     if (s->dsig) {
-        double gs = e * (6.0 * s6 - 3.0 * s3) / sg;
+        double gs = e * (6.0 * s6 - 3.0 * s->c * s3) / sg;
         s->dsig[ti] += gs;
         s->dsig[tj] += gs;
     }
@@ -355,11 +362,12 @@ static void sc_pair(void *c, int64_t i, int64_t j, const double *xi, const doubl
  * (all particles of type 0).  forces/deps/dsig may be NULL.  mode as orc_pairs.
  */
 int orc_softcore(const orc_box *in, const double *x, int64_t n, const int32_t *types, int ntypes,
-                 const double *eps, const double *sig, int mode, double *energy, double *forces,
+                 const double *eps, const double *sig, int mode, int form, double *energy, double *forces,
                  double *deps, double *dsig, int64_t *npairs) {
     if (in->dtype != 1) return ORC_ERR_DTYPE;
+    if (form != 0 && form != 1) return ORC_ERR_ARG;
     Box_f64 b; COPY_BOX_IN(in, b, double);
-    sc_ctx s = {types, eps, sig, 0.0, forces, deps, dsig, b.dim, b.invrot, 0};
+    sc_ctx s = {types, eps, sig, 0.0, forces, deps, dsig, b.dim, b.invrot, 0, form ? 0.0 : 1.0, form ? 2.0 : 1.0};
     if (forces) memset(forces, 0, sizeof(double) * n * b.dim);
     if (deps) memset(deps, 0, sizeof(double) * ntypes);
     if (dsig) memset(dsig, 0, sizeof(double) * ntypes);

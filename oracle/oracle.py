@@ -70,7 +70,7 @@ def lib():
         L.orc_cl_stats.argtypes = [C.POINTER(Box), P, I64, P]
         L.orc_pairs.argtypes = [C.POINTER(Box), P, I64, I, P, P, P, P, I64]
         L.orc_pairs.restype = I64
-        L.orc_softcore.argtypes = [C.POINTER(Box), P, I64, P, I, P, P, I, P, P, P, P, P]
+        L.orc_softcore.argtypes = [C.POINTER(Box), P, I64, P, I, P, P, I, I, P, P, P, P, P]
         L.orc_histogram.argtypes = [C.POINTER(Box), P, I64, P, I, D, I, P, P]
         L.orc_mindist.argtypes = [C.POINTER(Box), P, I64, I, P, P, P]
         L.orc_reduce.argtypes = [C.POINTER(Box), P, I64, I, P]
@@ -171,8 +171,10 @@ def canonical_pairs(i, j):
     return k[order], order
 
 
-def softcore(box: Box, x, eps=0.2, sigma=3.0, types=None, mode=0, forces=True, grads=False):
-    """FP64 soft-core energy (and forces, parameter gradients).  box must be float64."""
+def softcore(box: Box, x, eps=0.2, sigma=3.0, types=None, mode=0, forces=True, grads=False, form=0):
+    """FP64 pair energy (and forces, parameter gradients).  box must be float64.
+    form 0: soft-core eps[s^6 - s^3], s = sigma^2/(d2+1); form 1: Lennard-Jones
+    eps[(sigma/d)^12 - 2(sigma/d)^6] (the NAMD comparison's r_min form)."""
     x = _x(box, x)
     n = x.shape[0]
     eps = np.atleast_1d(np.asarray(eps, np.float64))
@@ -184,7 +186,7 @@ def softcore(box: Box, x, eps=0.2, sigma=3.0, types=None, mode=0, forces=True, g
     de = np.zeros(nt) if grads else None
     ds = np.zeros(nt) if grads else None
     npairs = np.zeros(1, np.int64)
-    rc = lib().orc_softcore(C.byref(box), _ptr(x), n, _ptr(t), nt, _ptr(eps), _ptr(sig), mode,
+    rc = lib().orc_softcore(C.byref(box), _ptr(x), n, _ptr(t), nt, _ptr(eps), _ptr(sig), mode, int(form),
                             _ptr(e), _ptr(f), _ptr(de), _ptr(ds), _ptr(npairs))
     if rc:
         raise OracleError(rc)
