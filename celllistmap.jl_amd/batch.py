@@ -85,8 +85,10 @@ class BatchedSystems:
             self._ctx.torch_after()
         self._xref = xs
 
-    def set_tuning(self, waves_per_cu: int):
+    def set_tuning(self, waves_per_cu: int, sweep_variant: Optional[int] = None):
         self._ctx.check(self._ctx.lib.clm_set_tuning(self._ctx.ptr, int(waves_per_cu)), "tuning")
+        if sweep_variant is not None:
+            self._ctx.check(self._ctx.lib.clm_set_sweep_variant(self._ctx.ptr, int(sweep_variant)), "sweep variant")
 
     def system_box(self, k: int) -> dict:
         from .core import _box_dict

@@ -475,6 +475,11 @@ class ParticleSystem:
         self._ny = n
         self._yref = ys
 
+    def set_sweep_variant(self, variant: int):
+        """0: each wave walks the union of its lanes' stencil ranges; 1: each lane walks
+        its own cutoff-bounded window of every stencil row.  Same results."""
+        self._ctx.check(self._ctx.lib.clm_set_sweep_variant(self._ctx.ptr, int(variant)), "sweep variant")
+
     def update_cell_list(self):
         """UpdateCellList! (CellLists.jl:727-927) on the device."""
         self._ctx.check(self._ctx.lib.clm_update_cell_list(self._ctx.ptr), "UpdateCellList!")

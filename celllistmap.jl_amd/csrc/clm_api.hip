@@ -55,6 +55,7 @@ struct clm_ctx {
     bool layout_dirty = true;
     DBuf sys_desc, sys_of, rowoff;
     int waves_per_cu = 0;  // persistent sweep grid; 0 = as many as fit
+    int sweep_variant = 0;  // 0 = union rows (k_sweep), 1 = per-lane windows (k_sweep_win)
 
     // captured pipeline (update + energy/forces) for replay
     hipGraph_t graph = nullptr;
@@ -70,7 +71,7 @@ struct clm_ctx {
     bool cl_valid = false;
     int ncells = 0, nrows = 0;
     int64_t n_entries = 0;
-    DBuf xw, home_cell, cnt, creal, cstart, cur_real, cur_ghost, E, flags;
+    DBuf xw, home_cell, cnt, creal, cstart, cur_real, cur_ghost, E, E_raw, rstart, real_pos, flags;
     int64_t e_cap = 0;
     DBuf hc, ntask, hstart[3], tstart[3];
     bool mode_valid[3] = {false, false, false};
@@ -363,7 +364,9 @@ template <typename T>
 clm_status ensure_bin_buffers(clm_ctx *c) {
     const int64_t n = c->n;
     const int nc1 = c->ncells + 1;
-    for (DBuf *b : {&c->cnt, &c->creal, &c->cstart, &c->cur_real, &c->cur_ghost}) HIPCHK(c, ensure(*b, sizeof(int) * nc1));
+    for (DBuf *b : {&c->cnt, &c->creal, &c->cstart, &c->cur_real, &c->cur_ghost, &c->rstart})
+        HIPCHK(c, ensure(*b, sizeof(int) * nc1));
+    HIPCHK(c, ensure(c->real_pos, sizeof(int) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->xw, sizeof(Ent<T>) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->home_cell, sizeof(int) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->flags, sizeof(uint32_t) * 4));
@@ -394,6 +397,7 @@ clm_status enqueue_count(clm_ctx *c) {
                                                         (int *)c->home_cell.p, (int *)c->cnt.p, (int *)c->creal.p,
                                                         (uint32_t *)c->flags.p));
     }
+    if ((s = scan_int(c, (const int *)c->creal.p, (int *)c->rstart.p, nc1))) return s;
     return scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1);
 }
 
@@ -413,15 +417,22 @@ clm_status enqueue_scatter(clm_ctx *c) {
                                                           (int)n, sd, sys_of_ptr(c), (const Ent<T> *)c->xw.p,
                                                           (const int *)c->home_cell.p, (const int *)c->cstart.p,
                                                           (const int *)c->creal.p, tp, (int *)c->cur_real.p,
-                                                          (int *)c->cur_ghost.p, (Ent<T> *)c->E.p, c->e_cap,
+                                                          (int *)c->cur_ghost.p, (Ent<T> *)c->E_raw.p, c->e_cap,
                                                           (uint32_t *)c->flags.p));
         else
             LAUNCH(c, "k_bin_scatter", hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
                                                           (int)n, sd, sys_of_ptr(c), (const Ent<T> *)c->xw.p,
                                                           (const int *)c->home_cell.p, (const int *)c->cstart.p,
                                                           (const int *)c->creal.p, tp, (int *)c->cur_real.p,
-                                                          (int *)c->cur_ghost.p, (Ent<T> *)c->E.p, c->e_cap,
+                                                          (int *)c->cur_ghost.p, (Ent<T> *)c->E_raw.p, c->e_cap,
                                                           (uint32_t *)c->flags.p));
+    }
+    // order every cell by x and list the real particles' positions (k_cell_sort)
+    if (c->ncells > 0) {
+        const int grid = std::max(1, std::min(c->ncells, c->num_cus * 32));
+        LAUNCH(c, "k_cell_sort", hipLaunchKernelGGL((k_cell_sort<T>), dim3(grid), dim3(64), 0, c->stream, c->ncells,
+                                                    (const int *)c->cstart.p, (const int *)c->rstart.p,
+                                                    (const Ent<T> *)c->E_raw.p, (Ent<T> *)c->E.p, (int *)c->real_pos.p));
     }
     return CLM_OK;
 }
@@ -454,7 +465,8 @@ clm_status update_cell_list_t(clm_ctx *c) {
     if ((s = check_flags(c, *c->h_flags))) return s;
     c->n_entries = total;
     HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
-    c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
+    HIPCHK(c, ensure(c->E_raw, c->E.bytes));
+    c->e_cap = (int64_t)(std::min(c->E.bytes, c->E_raw.bytes) / sizeof(Ent<T>)) - SWEEP_PAD;
     if ((s = enqueue_scatter<T>(c))) return s;
     c->cl_valid = true;
     c->dirty = false;
@@ -491,6 +503,7 @@ clm_status ensure_cross(clm_ctx *c) {
         c->sub_epoch = c->box_epoch;
     }
     y->waves_per_cu = c->waves_per_cu;
+    y->sweep_variant = c->sweep_variant;
     y->stream = c->stream;
     if ((s = ensure_cell_list(y))) return fail(c, s, y->err);
     return CLM_OK;
@@ -540,6 +553,7 @@ SweepArgs<T> sweep_args(clm_ctx *c, int mode) {
     SweepArgs<T> sa;
     sa.hstart = (const int *)c->hstart[mode].p;
     sa.tstart = (const int *)c->tstart[mode].p;
+    sa.real_pos = (const int *)c->real_pos.p;
     sa.rowoff = (const int *)c->rowoff.p;
     sa.sys = (const SysDesc<T> *)c->sys_desc.p;
     sa.task_counter = (int *)c->counters.p;
@@ -559,6 +573,14 @@ int occupancy_grid(clm_ctx *c, KernelT kernel) {
 }
 int sweep_grid(clm_ctx *c) { return std::max(64, c->num_cus * 16); }
 
+// Persistent grid of the sweep kernel variant the context uses.
+template <typename T, int MODE, typename Op>
+int sweep_occupancy(clm_ctx *c) {
+    if (c->sweep_variant == 1)
+        return c->dim == 3 ? occupancy_grid(c, k_sweep_win<T, MODE, 3, Op>) : occupancy_grid(c, k_sweep_win<T, MODE, 2, Op>);
+    return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE, 2, Op>);
+}
+
 // Sweep with homes from context c and neighbours from context nb (nb == c for one set).
 template <typename T, int MODE, typename Op>
 clm_status run_sweep_on(clm_ctx *c, clm_ctx *nb, const char *name, const Op &op, int grid) {
@@ -566,15 +588,22 @@ clm_status run_sweep_on(clm_ctx *c, clm_ctx *nb, const char *name, const Op &op,
     clm_status s;
     if ((s = fill_int(c, c->counters.p, 16, 0))) return s;
     SweepArgs<T> sa = sweep_args<T>(c, MODE);
-    if (grid <= 0) grid = c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE, 2, Op>);
+    if (grid <= 0) grid = sweep_occupancy<T, MODE, Op>(c);
     const Ent<T> *E = (const Ent<T> *)c->E.p;
     const int *cs = (const int *)c->cstart.p;
     const Ent<T> *En = (const Ent<T> *)nb->E.p;
     const int *csn = (const int *)nb->cstart.p;
-    if (c->dim == 3)
-        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
-    else
-        LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
+    if (c->sweep_variant == 1) {
+        if (c->dim == 3)
+            LAUNCH(c, name, hipLaunchKernelGGL((k_sweep_win<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
+        else
+            LAUNCH(c, name, hipLaunchKernelGGL((k_sweep_win<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
+    } else {
+        if (c->dim == 3)
+            LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
+        else
+            LAUNCH(c, name, hipLaunchKernelGGL((k_sweep<T, MODE, 2, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
+    }
     return CLM_OK;
 }
 
@@ -609,14 +638,10 @@ clm_status run_mode_a(clm_ctx *c, const char *name, const Op &op, int grid) {
 template <typename T, typename Op>
 int once_grid(clm_ctx *c) {
     switch (mode_once(c)) {
-        case MODE_A_TRIC:
-            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_A_TRIC, 2, Op>);
-        case MODE_X_ORTHO:
-            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_X_ORTHO, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_X_ORTHO, 2, Op>);
-        case MODE_X_REAL:
-            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_X_REAL, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_X_REAL, 2, Op>);
-        default:
-            return c->dim == 3 ? occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 3, Op>) : occupancy_grid(c, k_sweep<T, MODE_A_ORTHO, 2, Op>);
+        case MODE_A_TRIC: return sweep_occupancy<T, MODE_A_TRIC, Op>(c);
+        case MODE_X_ORTHO: return sweep_occupancy<T, MODE_X_ORTHO, Op>(c);
+        case MODE_X_REAL: return sweep_occupancy<T, MODE_X_REAL, Op>(c);
+        default: return sweep_occupancy<T, MODE_A_ORTHO, Op>(c);
     }
 }
 
@@ -811,7 +836,8 @@ clm_status graph_capture_t(clm_ctx *c, const clm_softcore *p, void *forces) {
     const int64_t want = c->n_entries + c->n_entries / 8 + 1024;
     if (c->e_cap < want) {
         HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * (want + SWEEP_PAD)));
-        c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
+        HIPCHK(c, ensure(c->E_raw, c->E.bytes));
+        c->e_cap = (int64_t)(std::min(c->E.bytes, c->E_raw.bytes) / sizeof(Ent<T>)) - SWEEP_PAD;
     }
     const int64_t nD = std::max<int64_t>(c->n * c->dim, 1);
     HIPCHK(c, ensure(c->outbuf, sizeof(T) * nD));
@@ -985,7 +1011,7 @@ clm_status clm_destroy(clm_ctx *c) {
                     &c->scan_tmp, &c->nl_i, &c->nl_j, &c->nl_d, &c->nl_count, &c->outbuf, &c->acc,
                     &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b, &c->sp_i, &c->sp_j, &c->sp_d,
                     &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart, &c->sys_desc,
-                    &c->sys_of, &c->rowoff};
+                    &c->sys_of, &c->rowoff, &c->E_raw, &c->rstart, &c->real_pos};
     for (DBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
@@ -1141,6 +1167,14 @@ clm_status clm_get_system_box(clm_ctx *c, int k, clm_box_info *out) {
 clm_status clm_set_tuning(clm_ctx *c, int waves_per_cu) {
     if (!c || waves_per_cu < 0) return CLM_ERR_ARG;
     c->waves_per_cu = waves_per_cu;
+    return CLM_OK;
+}
+
+clm_status clm_set_sweep_variant(clm_ctx *c, int variant) {
+    if (!c) return CLM_ERR_ARG;
+    if (variant != 0 && variant != 1) return fail(c, CLM_ERR_ARG, "sweep variant must be 0 or 1");
+    c->sweep_variant = variant;
+    if (c->ysub) c->ysub->sweep_variant = variant;
     return CLM_OK;
 }
 

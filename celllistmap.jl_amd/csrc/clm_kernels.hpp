@@ -153,6 +153,44 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, const SysDesc<T> *__
     if (flags) atomicOr(gflags, flags);
 }
 
+// Orders the entries of every cell by (x, w) -- x in the aligned frame, w breaking ties
+// (unique per entry) -- and records where each real particle landed: real_pos[rstart[c]
+// + k] = position of the k-th real of cell c.  Cells are contiguous intervals of x along a
+// row of the grid, so a whole row of cells is then ordered by x (up to rounding at the two
+// cells the border rule moves reals into), which lets the sweep stop a lane's walk along a
+// row as soon as it passes the end of that lane's window.  The order is also independent
+// of the atomic arrival order of the scatter.  One wave per cell; ranks by counting.
+template <typename T>
+__global__ void __launch_bounds__(64) k_cell_sort(int ncells, const int *__restrict__ cstart,
+                                                  const int *__restrict__ rstart, const Ent<T> *__restrict__ Ein,
+                                                  Ent<T> *__restrict__ Eout, int *__restrict__ real_pos) {
+    const int lane = threadIdx.x;
+    for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
+        const int s0 = cstart[c];
+        const int n = cstart[c + 1] - s0;
+        if (n == 0) continue;
+        const int r0 = rstart[c];
+        for (int k0 = 0; k0 < n; k0 += 64) {
+            const int k = k0 + lane;
+            const bool mine = k < n;
+            const Ent<T> e = Ein[s0 + (mine ? k : 0)];
+            const uint32_t we = ent_w(e);
+            int rank = 0, rrank = 0;
+            for (int m = 0; m < n; ++m) {
+                const Ent<T> f = Ein[s0 + m];
+                const uint32_t wf = ent_w(f);
+                const bool less = (f.x < e.x) || (f.x == e.x && wf < we);
+                rank += less ? 1 : 0;
+                rrank += (less && !(wf & GHOST_BIT)) ? 1 : 0;
+            }
+            if (mine) {
+                Eout[s0 + rank] = e;
+                if (!(we & GHOST_BIT)) real_pos[r0 + rrank] = s0 + rank;
+            }
+        }
+    }
+}
+
 // hc[c] = homes of cell c for the sweep mode; the trailing element is 0 for the scan.
 __global__ void k_home_counts(int ncells, int all_entries, const int *__restrict__ cstart,
                               const int *__restrict__ creal, int *__restrict__ hc) {
@@ -199,6 +237,7 @@ template <typename T>
 struct SweepArgs {
     const int *hstart;
     const int *tstart;
+    const int *real_pos;       // entry position of every real particle, in cell order (real-home modes)
     const int *rowoff;         // [nsys + 1] first global row of each system
     const SysDesc<T> *sys;
     int *task_counter;         // [8] per-XCD counters, zeroed before launch
@@ -262,7 +301,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
         const int h = hb + lane;
         const bool valid = h < he;
         const int c = last_le(sa.hstart, rowc0, rowc0 + ncx, valid ? h : hb);
-        const int pos = cs[c] + ((valid ? h : hb) - sa.hstart[c]);
+        const int pos = mode_all_entries(MODE) ? cs[c] + ((valid ? h : hb) - sa.hstart[c]) : sa.real_pos[valid ? h : hb];
         const Ent<T> p = E[pos];
         const int cx = c - rowc0;
         const int cy = lr % ncy, cz = lr / ncy;
@@ -325,6 +364,178 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                     }
 #pragma unroll
                     for (int u = 0; u < B; ++u) cur[u] = nxt[u];
+                }
+            }
+        }
+        op.lane_end(sh, ws, ls, p, valid, sd, si);
+    }
+    op.wave_end(sh, ws);
+}
+
+// ------------------------------------------------------------------------------
+// windowed sweep
+// ------------------------------------------------------------------------------
+//
+// Same tasks, modes and operators as k_sweep, but each lane walks only its own window
+// of every stencil row instead of the union of the 64 lanes' ranges.  For a home p and
+// the row (dy, dz), let g be the distance from p to the row's slab in y and z; only
+// points with |x - p.x| <= h = sqrt(rc^2 - g^2) can be within the cutoff.  The window
+// starts at the first entry of the cell holding p.x - h and, the row being ordered by x
+// (k_cell_sort), ends at the first entry with x > p.x + h.  Every lane steps through
+// its window in lock-step with the others (the trip count is the longest window of the
+// wave), loading its own neighbour entries.  Which entries are visited never decides a
+// pair -- the d2 <= rc^2 test and the mode filter do -- so the bounds are taken with a
+// margin of 1e-3 cell sizes, far above the rounding of the quantities involved, and the
+// two grid cells into which the border rule moves real particles are widened by a cell.
+
+__device__ inline int wave_max_int(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return uniform_int(v);
+}
+
+// Distance from coordinate v to the slab of cell k along axis a (0 inside), with the
+// border-rule widening and the margin; inv = 1/cs[a].
+template <typename T>
+__device__ inline T slab_gap(const DevBox<T> &b, int a, int k, T v, T margin) {
+    T lo = b.cbmin[a] + T(k) * b.cs[a] - margin;
+    T hi = b.cbmin[a] + T(k + 1) * b.cs[a] + margin;
+    if (k == b.lcell) lo -= b.cs[a];
+    if (k == b.nc[a] - b.lcell - 1) hi += b.cs[a];
+    T g = lo - v;
+    const T g2 = v - hi;
+    g = g2 > g ? g2 : g;
+    return g > T(0) ? g : T(0);
+}
+
+template <typename T>
+__device__ inline int axis_cell(const DevBox<T> &b, int a, T v, T inv) {
+    T f = dev_floor((v - b.cbmin[a]) * inv);
+    const T top = T(b.nc[a]);
+    f = f < T(-1) ? T(-1) : (f > top ? top : f);
+    return (int)f;
+}
+
+template <typename T>
+__device__ inline T window_sqrt(T v);
+template <>
+__device__ inline float window_sqrt<float>(float v) { return __builtin_amdgcn_sqrtf(v); }
+template <>
+__device__ inline double window_sqrt<double>(double v) { return __builtin_sqrt(v); }
+
+template <typename T, int MODE, int D, typename Op>
+__global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, const int *__restrict__ cs,
+                                                  const Ent<T> *__restrict__ En, const int *__restrict__ csn,
+                                                  const SweepArgs<T> sa, const Op op) {
+    __shared__ typename Op::Shared sh;
+    __shared__ int s_task;
+    const int lane = threadIdx.x;
+    constexpr int U = 4;  // entries per lane in flight
+    typename Op::Wave ws;
+    op.wave_init(sh, ws);
+    const int ntasks = sa.tstart[sa.nrows];
+    int home_xcd = blockIdx.x & 7, probe = 0;
+    while (true) {
+        if (lane == 0) s_task = fetch_task(sa.task_counter, ntasks, home_xcd, probe);
+        __syncthreads();
+        const int task = uniform_int(s_task);
+        __syncthreads();
+        if (task < 0) break;
+        const int r = last_le(sa.tstart, 0, sa.nrows + 1, task);
+        const int si = uniform_int(last_le(sa.rowoff, 0, sa.nsys + 1, r));
+        const SysDesc<T> &sd = sa.sys[si];
+        const DevBox<T> &bx = sd.box;
+        const int ncx = bx.nc[0], ncy = bx.nc[1];
+        const int L = bx.lcell;
+        const T rc2 = bx.rc2;
+        const int lr = r - sa.rowoff[si];
+        const int chunk = task - sa.tstart[r];
+        const int rowc0 = sd.cell_off + lr * ncx;
+        const int hb = sa.hstart[rowc0] + chunk * 64;
+        const int he = sa.hstart[rowc0 + ncx];
+        const int h = hb + lane;
+        const bool valid = h < he;
+        const int c = last_le(sa.hstart, rowc0, rowc0 + ncx, valid ? h : hb);
+        const int pos = mode_all_entries(MODE) ? cs[c] + ((valid ? h : hb) - sa.hstart[c]) : sa.real_pos[valid ? h : hb];
+        const Ent<T> p = E[pos];
+        const int cx = c - rowc0;
+        const int cy = lr % ncy, cz = lr / ncy;
+        const uint32_t wp = ent_w(p);
+        const uint32_t ip = wp & IDX_MASK;
+        const bool preal = !(wp & GHOST_BIT);
+        const T mx = T(1e-3) * bx.cs[0], my = T(1e-3) * bx.cs[1], mz = T(1e-3) * bx.cs[2];
+        const T invx = T(1) / bx.cs[0];
+        typename Op::Lane ls;
+        op.lane_init(ws, ls, p, valid);
+        const int dzr = (D == 3) ? L : 0;
+        for (int dz = -dzr; dz <= dzr; ++dz) {
+            const T gz = (D == 3) ? slab_gap(bx, 2, cz + dz, p.z, mz) : T(0);
+            for (int dy = -L; dy <= L; ++dy) {
+                const int base = sd.cell_off + ((cz + dz) * ncy + (cy + dy)) * ncx;
+                const bool selfrow = (dz == 0 && dy == 0);
+                int xlo = -L;
+                if (MODE == MODE_A_ORTHO) xlo = (dy > 0 || (dy == 0 && dz > 0)) ? 0 : 1;
+                const int k0 = cx + ((MODE == MODE_A_ORTHO && selfrow) ? 0 : xlo);
+                const int k1 = cx + L;
+                const T gy = slab_gap(bx, 1, cy + dy, p.y, my);
+                const T r2 = rc2 - gy * gy - gz * gz;
+                int j0 = 0, len = 0;
+                T xend = p.x;
+                if (valid && r2 >= T(0) && k0 <= k1) {
+                    const T hw = window_sqrt(r2) + mx;
+                    int ka = axis_cell(bx, 0, p.x - hw, invx);
+                    int kb = axis_cell(bx, 0, p.x + hw, invx);
+                    if (kb == L - 1) kb = L;
+                    if (ka == ncx - L) ka = ncx - L - 1;
+                    ka = max(ka, k0);
+                    kb = min(kb, k1);
+                    if (ka <= kb) {
+                        j0 = csn[base + ka];
+                        int j1 = csn[base + kb + 1];
+                        if (MODE == MODE_A_ORTHO && selfrow) j0 = max(j0, pos + 1);
+                        len = max(j1 - j0, 0);
+                        xend = p.x + hw + mx;
+                    }
+                }
+                // MODE_X_ORTHO: entries of the same-index cell pair only with a real home
+                int slo = 0, shi = 0;
+                if (MODE == MODE_X_ORTHO && selfrow) { slo = csn[base + cx]; shi = csn[base + cx + 1]; }
+                const int maxlen = wave_max_int(len);
+                Ent<T> cur[U], nxt[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) cur[u] = En[u < len ? j0 + u : j0];
+                for (int it = 0; it < maxlen; it += U) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) nxt[u] = En[it + U + u < len ? j0 + it + U + u : j0];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int t = it + u;
+                        const Ent<T> &q = cur[u];
+                        bool act = t < len;
+                        if (act && q.x > xend) {  // the row is ordered by x: the window is over
+                            len = t;
+                            act = false;
+                        }
+                        const uint32_t wq = ent_w(q);
+                        const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
+                        T d2 = dx * dx + dy2 * dy2;
+                        if (D == 3) d2 = d2 + dz2 * dz2;
+                        bool ok = act && d2 <= rc2;
+                        if (MODE == MODE_A_ORTHO) {
+                            ok = ok && (preal || !(wq & GHOST_BIT));
+                        } else if (MODE == MODE_A_TRIC) {
+                            ok = ok && (ip < (wq & IDX_MASK));
+                        } else if (MODE == MODE_X_ORTHO) {
+                            const int j = j0 + t;
+                            const bool same = (j >= slo) && (j < shi);
+                            ok = ok && (preal || (!same && !(wq & GHOST_BIT)));
+                        } else if (MODE == MODE_B) {
+                            ok = ok && (ip != (wq & IDX_MASK));
+                        }
+                        op.candidate(sh, ws, ls, ok, p, q, d2);
+                    }
+                    if (ballot64(it + U < len) == 0) break;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) cur[u] = nxt[u];
                 }
             }
         }

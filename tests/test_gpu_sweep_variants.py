@@ -1,0 +1,105 @@
+"""The two sweep kernels (union rows / per-lane windows) against the oracle on the same
+inputs: pair sets and distances bit-exact, energies and forces to the usual tolerance,
+histogram counts and minimum distances exact.  Includes the pathological boundary
+coordinates and the border-rule cells, where the windowed kernel's bounds are widened."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def canon(res):
+    a, b, d = res.canonical()
+    return a * (1 << 32) + b, d
+
+
+def oracle_canon(O, box, x):
+    i, j, d2, d = O.pairs(box, x, 1)
+    k, order = O.canonical_pairs(i, j)
+    return k, d[order]
+
+
+def boxes():
+    rng = np.random.default_rng(4321)
+    L = np.array([21.0, 24.5, 27.0])
+    yield "ortho", L, rng.random((2500, 3)) * L
+    M = np.array([[26.0, 6.0, 3.0], [0.0, 24.0, 5.0], [0.0, 0.0, 23.0]])
+    yield "tric", M, (M @ rng.random((3, 2500))).T
+    # points on the faces / corners of the unit cell (border rule, p == 1 wrap)
+    side = 16.0
+    f = rng.integers(0, 5, (1500, 3)) / 4.0
+    yield "faces", np.array([side, side, side]), f * side + rng.normal(0, 1e-6, (1500, 3))
+    M2 = np.array([[18.0, 4.0], [0.0, 17.0]])
+    yield "tric2d", M2, (M2 @ rng.random((2, 900))).T
+    yield "nonperiodic", None, rng.random((2000, 3)) * [30.0, 20.0, 25.0]
+
+
+CASES = list(boxes())
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("lcell", [1, 2, 3])
+@pytest.mark.parametrize("idx", range(len(CASES)))
+def test_variant_neighborlist(clm, O, variant, dtype, lcell, idx):
+    name, uc, x = CASES[idx]
+    say(f"[variant {variant}] nl {name} {np.dtype(dtype).name} lcell={lcell}")
+    x = x.astype(dtype)
+    rc = 3.7
+    sys_ = clm.ParticleSystem(xpositions=x, unitcell=uc, cutoff=rc, lcell=lcell, output=clm.NeighborListOp())
+    sys_.set_sweep_variant(variant)
+    kg, dg = canon(clm.pairwise_(None, sys_))
+    box = O.make_box(uc, rc, lcell=lcell, dtype=dtype) if uc is not None else \
+        O.make_nonperiodic_box(x, rc, lcell=lcell, dtype=dtype)
+    ko, do = oracle_canon(O, box, x)
+    assert np.array_equal(kg, ko)
+    assert np.array_equal(dg.astype(np.float64), do)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_variant_operators(clm, O, variant, idx):
+    name, uc, x = CASES[idx]
+    say(f"[variant {variant}] operators {name}")
+    rc = 4.1
+    sys_ = clm.ParticleSystem(xpositions=x, unitcell=uc, cutoff=rc, lcell=2)
+    sys_.set_sweep_variant(variant)
+    box = O.make_box(uc, rc)
+    r = clm.pairwise_(clm.SoftCoreEnergyForces(0.2, 3.0), sys_)
+    ref = O.softcore(box, x, 0.2, 3.0)
+    assert abs(r.energy - ref["energy"]) <= 1e-10 * abs(ref["energy"])
+    assert np.linalg.norm(r.forces - ref["forces"]) <= 1e-10 * np.linalg.norm(ref["forces"])
+    counts = clm.pairwise_(clm.DistanceHistogram(nbins=20, width=0.2), sys_)
+    oc, _ = O.histogram(box, x, 20, 0.2, mode=1)
+    assert np.array_equal(counts, oc)
+    i, j, d = clm.pairwise_(clm.MinimumDistance(), sys_)
+    assert (i, j, d) == O.mindist(box, x, 1)
+    g = clm.energy_adjoint(sys_, 0.2, 3.0)
+    ga = O.softcore(box, x, 0.2, 3.0, grads=True)
+    assert abs(g.deps[0] - ga["deps"][0]) <= 1e-10 * abs(ga["deps"][0])
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("tric", [False, True])
+def test_variant_cross(clm, O, variant, tric):
+    say(f"[variant {variant}] cross tric={tric}")
+    rng = np.random.default_rng(7 + tric)
+    M = np.array([[22.0, 4.0, 2.0], [0.0, 25.0, 3.0], [0.0, 0.0, 21.0]]) if tric else np.array([22.0, 25.0, 21.0])
+    Mm = M if tric else np.diag(M)
+    x = (Mm @ rng.random((3, 1200))).T
+    y = (Mm @ rng.random((3, 1700))).T
+    sys_ = clm.ParticleSystem(xpositions=x, ypositions=y, unitcell=M, cutoff=3.9, lcell=2, output=clm.NeighborListOp())
+    sys_.set_sweep_variant(variant)
+    ig, jg, dg = clm.pairwise_(None, sys_).ordered()
+    i, j, d2, d = O.pairs_cross(O.make_box(M, 3.9, lcell=2), x, y, 1)
+    o = np.lexsort((j, i))
+    assert np.array_equal(ig, i[o]) and np.array_equal(jg, j[o]) and np.array_equal(dg, d[o])
+    r = clm.pairwise_(clm.SoftCoreEnergyForces(0.2, 3.0), sys_)
+    b = O.make_box(M, 3.9)
+    a = O.softcore(b, np.concatenate([x, y]), 0.2, 3.0)
+    e = a["energy"] - O.softcore(b, x, 0.2, 3.0)["energy"] - O.softcore(b, y, 0.2, 3.0)["energy"]
+    assert abs(r.energy - e) <= 1e-9 * abs(a["energy"])
