@@ -13,14 +13,21 @@ def say(*a):
 
 
 def canon(res):
+    """(key, d) sorted by key, then d: a pair the reference's sweep reports twice (it does
+    at float32 for points within rounding of the cell faces, see the faces case) then
+    compares as a multiset."""
     a, b, d = res.canonical()
-    return a * (1 << 32) + b, d
+    k = a * (1 << 32) + b
+    o = np.lexsort((d.astype(np.float64), k))
+    return k[o], d[o]
 
 
 def oracle_canon(O, box, x):
     i, j, d2, d = O.pairs(box, x, 1)
-    k, order = O.canonical_pairs(i, j)
-    return k, d[order]
+    k, _ = O.canonical_pairs(i, j)
+    k = np.minimum(i, j).astype(np.int64) * (1 << 32) + np.maximum(i, j).astype(np.int64)
+    o = np.lexsort((d, k))
+    return k[o], d[o]
 
 
 def boxes():
@@ -72,7 +79,12 @@ def test_variant_operators(clm, O, variant, idx):
     r = clm.pairwise_(clm.SoftCoreEnergyForces(0.2, 3.0), sys_)
     ref = O.softcore(box, x, 0.2, 3.0)
     assert abs(r.energy - ref["energy"]) <= 1e-10 * abs(ref["energy"])
-    assert np.linalg.norm(r.forces - ref["forces"]) <= 1e-10 * np.linalg.norm(ref["forces"])
+    # The faces case holds pairs 1e-6 apart across the periodic boundary: their pair
+    # vector, taken from a different (equally valid) image copy, carries an absolute
+    # rounding of ulp(16) ~ 4e-15, i.e. 4e-9 of the vector, so forces are compared there
+    # at 1e-8.
+    ftol = 1e-8 if name == "faces" else 1e-10
+    assert np.linalg.norm(r.forces - ref["forces"]) <= ftol * np.linalg.norm(ref["forces"])
     counts = clm.pairwise_(clm.DistanceHistogram(nbins=20, width=0.2), sys_)
     oc, _ = O.histogram(box, x, 20, 0.2, mode=1)
     assert np.array_equal(counts, oc)
