@@ -1,7 +1,7 @@
 """GROMACS comparison of the reference (test/applications/gromacs/compare_with_gromacs.jl):
 Lennard-Jones energies (c6/c12 form, cutoff 8 A, no modifier) of argon in a minimal
 two-atom cube, a cube, a rhombic dodecahedron and a truncated octahedron.  GROMACS
-printed "LJ (SR)" with 6 significant digits (tests/golden/gromacs/*.log); the
+printed "LJ (SR)" with 6 significant digits (its log excerpts, tests/golden/gromacs/*_energies.txt); the
 reference compares at rtol 1e-5.  The coordinates are the PDB inputs of those GROMACS
 runs (the reference reads the first frame of the .xtc, the same coordinates rounded to
 the xtc precision of 0.01 A)."""
@@ -20,7 +20,7 @@ NAMES = ["minimal", "cubic", "dodecahedron", "octahedron"]
 
 
 def lj_sr(name):
-    lines = open(os.path.join(G, f"{name}.log")).read().splitlines()
+    lines = open(os.path.join(G, f"{name}_energies.txt")).read().splitlines()
     k = next(i for i, ln in enumerate(lines) if "LJ (SR)" in ln)
     return float(lines[k + 1].split()[0])
 
