@@ -185,10 +185,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--lcell", type=int, default=2)
+    ap.add_argument("--lcell", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ef-reps", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint comparisons")
+    ap.add_argument("--sweep", type=int, default=1, help="sweep kernel: 0 union rows, 1 per-lane windows")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -217,7 +218,8 @@ def main():
 
     nl = clm.InPlaceNeighborList(xpositions=x, cutoff=rc, unitcell=M, lcell=args.lcell, device_output=True)
     ctx = nl.sys._ctx
-    log(f"[bench] N={args.n} lcell={args.lcell} box nc={nl.sys.box['nc']}")
+    nl.sys.set_sweep_variant(args.sweep)
+    log(f"[bench] N={args.n} lcell={args.lcell} sweep={args.sweep} box nc={nl.sys.box['nc']}")
 
     def step(k):
         x.add_(disp[k])
@@ -244,8 +246,8 @@ def main():
     log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step, {P:.4g} pairs")
 
     kernels = {}
-    for name in ("k_sweep_neighborlist", "k_bin_count", "k_bin_scatter", "scan", "k_fill_int",
-                 "k_home_counts", "k_row_tasks"):
+    for name in ("k_sweep_neighborlist", "k_bin_count", "k_bin_scatter", "k_cell_sort", "scan", "k_fill_int",
+                 "k_home_counts", "k_row_tasks", "k_nl_holes", "k_nl_fill"):
         tot, cnt = kernel_stats(ctx, name)
         if cnt:
             kernels[name] = {"avg_ms": tot / cnt, "launches": cnt, "ms_per_step": tot / args.steps}
@@ -320,7 +322,7 @@ def main():
         "config": {"workload": "single large system: N=1,000,000 float32, triclinic cell "
                                "[[100,20,10],[0,100,15],[0,0,100]]*(N/1e5)^(1/3), cutoff 12; step = "
                                "update!(positions) + UpdateCellList! + neighborlist! (InPlaceNeighborList)",
-                   "N": args.n, "cutoff": rc, "lcell": args.lcell, "pairs_per_step": P,
+                   "N": args.n, "cutoff": rc, "lcell": args.lcell, "sweep_variant": args.sweep, "pairs_per_step": P,
                    "entries_incl_ghosts": n_entries},
         "roofline": roofline,
         "cpu_baseline": cpu,

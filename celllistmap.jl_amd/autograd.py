@@ -57,8 +57,9 @@ def _make_function():
                 dedx = torch.as_tensor(g.dedx, device=pdev)
                 deps = torch.as_tensor(g.deps, device=edev, dtype=edt)
                 dsig = torch.as_tensor(g.dsigma, device=sdev, dtype=sdt)
-            ge = grad_e.to(dedx.dtype)
-            return (ge * dedx, grad_e.to(deps.dtype) * deps, grad_e.to(dsig.dtype) * dsig, None, None, None)
+            ge = grad_e.to(device=dedx.device, dtype=dedx.dtype)
+            return (ge * dedx, grad_e.to(device=deps.device, dtype=deps.dtype) * deps,
+                    grad_e.to(device=dsig.device, dtype=dsig.dtype) * dsig, None, None, None)
 
     return SoftCoreEnergyFunction
 

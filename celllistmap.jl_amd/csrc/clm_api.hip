@@ -55,7 +55,7 @@ struct clm_ctx {
     bool layout_dirty = true;
     DBuf sys_desc, sys_of, rowoff;
     int waves_per_cu = 0;  // persistent sweep grid; 0 = as many as fit
-    int sweep_variant = 0;  // 0 = union rows (k_sweep), 1 = per-lane windows (k_sweep_win)
+    int sweep_variant = 1;  // 0 = union rows (k_sweep), 1 = per-lane windows (k_sweep_win)
 
     // captured pipeline (update + energy/forces) for replay
     hipGraph_t graph = nullptr;

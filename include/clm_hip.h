@@ -200,8 +200,8 @@ clm_status clm_graph_launch(clm_ctx *ctx, double *energies);
 clm_status clm_set_tuning(clm_ctx *ctx, int waves_per_cu);
 /* Sweep kernel: 0 = every lane walks the union of its wave's stencil ranges (wave-uniform
  * neighbour loads); 1 = every lane walks its own window of each stencil row, bounded by
- * the cutoff sphere (per-lane loads).  Same pairs and results either way; an autotuner
- * knob with no reference counterpart. */
+ * the cutoff sphere (per-lane loads; the default).  Same pairs and results either way;
+ * an autotuner knob with no reference counterpart. */
 clm_status clm_set_sweep_variant(clm_ctx *ctx, int variant);
 
 /* ---- measurement ----------------------------------------------------------- */
