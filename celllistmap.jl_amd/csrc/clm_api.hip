@@ -71,7 +71,9 @@ struct clm_ctx {
     bool cl_valid = false;
     int ncells = 0, nrows = 0;
     int64_t n_entries = 0;
-    DBuf xw, home_cell, cnt, creal, cstart, cur_real, cur_ghost, E, E_raw, rstart, real_pos, flags;
+    DBuf xw, home_cell, cnt, creal, cstart, cur_real, cur_ghost, E, E_raw, rstart, real_pos, order, flags, cnt64, maxcnt;
+    bool order_valid = false;  // order[] holds the particles in the cell order of the last build
+    int bucket = 0;  // entries per cell bucket of the single-pass binning; 0 = not sized yet, -1 = off
     int64_t e_cap = 0;
     DBuf hc, ntask, hstart[3], tstart[3];
     bool mode_valid[3] = {false, false, false};
@@ -367,6 +369,7 @@ clm_status ensure_bin_buffers(clm_ctx *c) {
     for (DBuf *b : {&c->cnt, &c->creal, &c->cstart, &c->cur_real, &c->cur_ghost, &c->rstart})
         HIPCHK(c, ensure(*b, sizeof(int) * nc1));
     HIPCHK(c, ensure(c->real_pos, sizeof(int) * std::max<int64_t>(n, 1)));
+    HIPCHK(c, ensure(c->order, sizeof(int) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->xw, sizeof(Ent<T>) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->home_cell, sizeof(int) * std::max<int64_t>(n, 1)));
     HIPCHK(c, ensure(c->flags, sizeof(uint32_t) * 4));
@@ -401,6 +404,20 @@ clm_status enqueue_count(clm_ctx *c) {
     return scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1);
 }
 
+// Order every cell by x into E and list the real particles' positions (k_cell_sort); the
+// unsorted entries are in E_raw, compact (bucket 0) or in cell buckets.
+template <typename T>
+clm_status enqueue_sort(clm_ctx *c, int bucket) {
+    if (c->ncells <= 0) return CLM_OK;
+    const int grid = std::max(1, std::min(c->ncells, c->num_cus * 32));
+    LAUNCH(c, "k_cell_sort", hipLaunchKernelGGL((k_cell_sort<T>), dim3(grid), dim3(64), 0, c->stream, c->ncells,
+                                                (const int *)c->cstart.p, (const int *)c->rstart.p,
+                                                (const Ent<T> *)c->E_raw.p, bucket, (Ent<T> *)c->E.p,
+                                                (int *)c->real_pos.p, (int *)c->order.p));
+    c->order_valid = true;
+    return CLM_OK;
+}
+
 // k_bin_scatter into E (capacity c->e_cap; overflow raises FLAG_OVERFLOW).
 template <typename T>
 clm_status enqueue_scatter(clm_ctx *c) {
@@ -427,20 +444,84 @@ clm_status enqueue_scatter(clm_ctx *c) {
                                                           (int *)c->cur_ghost.p, (Ent<T> *)c->E_raw.p, c->e_cap,
                                                           (uint32_t *)c->flags.p));
     }
-    // order every cell by x and list the real particles' positions (k_cell_sort)
-    if (c->ncells > 0) {
-        const int grid = std::max(1, std::min(c->ncells, c->num_cus * 32));
-        LAUNCH(c, "k_cell_sort", hipLaunchKernelGGL((k_cell_sort<T>), dim3(grid), dim3(64), 0, c->stream, c->ncells,
-                                                    (const int *)c->cstart.p, (const int *)c->rstart.p,
-                                                    (const Ent<T> *)c->E_raw.p, (Ent<T> *)c->E.p, (int *)c->real_pos.p));
-    }
-    return CLM_OK;
+    return enqueue_sort<T>(c, 0);
 }
 
 clm_status check_flags(clm_ctx *c, uint32_t fl) {
     if (fl & FLAG_NAN) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN or Inf)");
     if (fl & FLAG_RANGE)
         return fail(c, CLM_ERR_RANGE, "a particle maps outside the computing grid (the reference raises a BoundsError)");
+    return CLM_OK;
+}
+
+// Entries per bucket for a fullest cell of mx entries: a power of two with 25 % headroom;
+// -1 (single-pass binning off) when the buckets would take more than sixteen times the
+// compact cell list and 1 GiB.
+int bucket_size(const clm_ctx *c, int mx, size_t esize) {
+    int64_t b = 8;
+    while (b < (int64_t)mx + mx / 4 + 4) b *= 2;
+    const double need = (double)c->ncells * (double)b * (double)esize;
+    const double limit = std::max(16.0 * (double)(c->n_entries + 1) * (double)esize, 1024.0 * 1024 * 1024);
+    if (need > limit || b > (1 << 20)) return -1;
+    return (int)b;
+}
+
+// Single-pass binning (k_bin_bucket + unpack + scans + k_cell_sort), one host
+// synchronisation.  done = false when the buckets cannot be sized within the memory
+// limit (the caller then bins in two passes).
+template <typename T>
+clm_status bin_bucketed(clm_ctx *c, bool &done) {
+    done = false;
+    const int64_t n = c->n;
+    const int nc1 = c->ncells + 1;
+    HIPCHK(c, ensure(c->cnt64, sizeof(unsigned long long) * std::max(c->ncells, 1)));
+    HIPCHK(c, ensure(c->maxcnt, sizeof(int) * 2));
+    const SysDesc<T> *sd = (const SysDesc<T> *)c->sys_desc.p;
+    const int *tp = c->has_types ? (const int *)c->types.p : nullptr;
+    const int *ord = c->order_valid ? (const int *)c->order.p : nullptr;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (c->bucket <= 0) return CLM_OK;
+        HIPCHK(c, ensure(c->E_raw, sizeof(Ent<T>) * (size_t)c->ncells * (size_t)c->bucket));
+        clm_status s;
+        if ((s = fill_int(c, c->cnt64.p, 2 * c->ncells, 0))) return s;
+        if ((s = fill_int(c, c->flags.p, 4, 0))) return s;
+        if ((s = fill_int(c, c->maxcnt.p, 1, 0))) return s;
+        if (n > 0) {
+            if (c->dim == 3)
+                LAUNCH(c, "k_bin_bucket", hipLaunchKernelGGL((k_bin_bucket<T, 3>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                                                             (const T *)c->pos.p, (int)n, ord, sd, sys_of_ptr(c), tp,
+                                                             (unsigned long long *)c->cnt64.p, (Ent<T> *)c->E_raw.p,
+                                                             c->bucket, (uint32_t *)c->flags.p));
+            else
+                LAUNCH(c, "k_bin_bucket", hipLaunchKernelGGL((k_bin_bucket<T, 2>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                                                             (const T *)c->pos.p, (int)n, ord, sd, sys_of_ptr(c), tp,
+                                                             (unsigned long long *)c->cnt64.p, (Ent<T> *)c->E_raw.p,
+                                                             c->bucket, (uint32_t *)c->flags.p));
+        }
+        LAUNCH(c, "k_bin_unpack", hipLaunchKernelGGL(k_bin_unpack, dim3(std::min<unsigned>(blocks(nc1, 256), 512)), dim3(256), 0,
+                                                     c->stream, c->ncells,
+                                                     (const unsigned long long *)c->cnt64.p, (int *)c->cnt.p,
+                                                     (int *)c->creal.p, (int *)c->maxcnt.p));
+        if ((s = scan_int(c, (const int *)c->creal.p, (int *)c->rstart.p, nc1))) return s;
+        if ((s = scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1))) return s;
+        int total = 0, mx = 0;
+        HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&mx, c->maxcnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        if ((s = sync(c))) return s;
+        if ((s = check_flags(c, *c->h_flags))) return s;
+        c->n_entries = total;
+        if (*c->h_flags & FLAG_OVERFLOW) {  // a cell outgrew its bucket: larger buckets, again
+            c->bucket = bucket_size(c, mx, sizeof(Ent<T>));
+            continue;
+        }
+        HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
+        c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
+        if ((s = enqueue_sort<T>(c, c->bucket))) return s;
+        done = true;
+        return CLM_OK;
+    }
+    c->bucket = -1;
     return CLM_OK;
 }
 
@@ -455,19 +536,36 @@ clm_status update_cell_list_t(clm_ctx *c) {
         c->layout_dirty = true;
         ++c->box_epoch;
     }
-    if (c->layout_dirty && (s = layout_systems<T>(c))) return s;
+    if (c->layout_dirty) {
+        if ((s = layout_systems<T>(c))) return s;
+        c->bucket = 0;  // re-sized on the first two-pass build of the new layout
+        c->order_valid = false;
+    }
     if ((s = ensure_bin_buffers<T>(c))) return s;
-    if ((s = enqueue_count<T>(c))) return s;
-    int total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    if ((s = sync(c))) return s;
-    if ((s = check_flags(c, *c->h_flags))) return s;
-    c->n_entries = total;
-    HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
-    HIPCHK(c, ensure(c->E_raw, c->E.bytes));
-    c->e_cap = (int64_t)(std::min(c->E.bytes, c->E_raw.bytes) / sizeof(Ent<T>)) - SWEEP_PAD;
-    if ((s = enqueue_scatter<T>(c))) return s;
+    bool done = false;
+    if (c->bucket > 0 && (s = bin_bucketed<T>(c, done))) return s;
+    if (!done) {
+        // two passes: count (+ the fullest cell, which sizes the buckets of later builds),
+        // scan, scatter
+        if ((s = enqueue_count<T>(c))) return s;
+        HIPCHK(c, ensure(c->maxcnt, sizeof(int) * 2));
+        if ((s = fill_int(c, c->maxcnt.p, 1, 0))) return s;
+        LAUNCH(c, "k_count_max", hipLaunchKernelGGL(k_count_max, dim3(std::min<unsigned>(blocks(c->ncells + 1, 256), 512)),
+                                                    dim3(256), 0, c->stream,
+                                                    c->ncells, (const int *)c->cnt.p, (int *)c->maxcnt.p));
+        int total = 0, mx = 0;
+        HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&mx, c->maxcnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        if ((s = sync(c))) return s;
+        if ((s = check_flags(c, *c->h_flags))) return s;
+        c->n_entries = total;
+        HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
+        HIPCHK(c, ensure(c->E_raw, c->E.bytes));
+        c->e_cap = (int64_t)(std::min(c->E.bytes, c->E_raw.bytes) / sizeof(Ent<T>)) - SWEEP_PAD;
+        if ((s = enqueue_scatter<T>(c))) return s;
+        if (c->bucket == 0) c->bucket = bucket_size(c, mx, sizeof(Ent<T>));
+    }
     c->cl_valid = true;
     c->dirty = false;
     for (bool &m : c->mode_valid) m = false;
@@ -593,6 +691,9 @@ clm_status run_sweep_on(clm_ctx *c, clm_ctx *nb, const char *name, const Op &op,
     const int *cs = (const int *)c->cstart.p;
     const Ent<T> *En = (const Ent<T> *)nb->E.p;
     const int *csn = (const int *)nb->cstart.p;
+    sa.en_bytes = (uint32_t)std::min<size_t>(nb->E.bytes, 0xFFFFFFF0u);
+    if (c->sweep_variant == 1 && nb->E.bytes > 0xFFFFFFF0u)
+        return fail(c, CLM_ERR_ARG, "cell list above 4 GiB: use sweep variant 0");
     if (c->sweep_variant == 1) {
         if (c->dim == 3)
             LAUNCH(c, name, hipLaunchKernelGGL((k_sweep_win<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
@@ -663,41 +764,60 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
     const int grid = once_grid<T, OpNeighborList<T>>(c);
     const int64_t ub_tasks = c->n_entries / 64 + c->nrows + 1;
     const int64_t active = std::min<int64_t>(grid, ub_tasks);
-    const int64_t scap = 2 * (int64_t)NL_CHUNK * active + NL_CHUNK;
-    const int64_t nchunks = (cap + scap + NL_CHUNK - 1) / NL_CHUNK + 1;
-    HIPCHK(c, ensure(c->sp_i, sizeof(int32_t) * scap));
-    HIPCHK(c, ensure(c->sp_j, sizeof(int32_t) * scap));
-    HIPCHK(c, ensure(c->sp_d, sizeof(T) * scap));
-    for (DBuf *b : {&c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart})
-        HIPCHK(c, ensure(*b, sizeof(int) * (nchunks + 1)));
-    HIPCHK(c, ensure(c->nl_count, sizeof(unsigned long long) * 2));
-    if ((s = fill_int(c, c->nl_count.p, 4, 0))) return s;
-    if ((s = fill_int(c, c->ch_used.p, nchunks + 1, 0))) return s;
-    NLOut<T> out;
-    out.oi = di; out.oj = dj; out.od = dd;
-    out.si = (int32_t *)c->sp_i.p; out.sj = (int32_t *)c->sp_j.p; out.sd = (T *)c->sp_d.p;
-    out.cap = cap;
-    out.scap = scap;
-    OpNeighborList<T> op;
-    std::memset((void *)&op, 0, sizeof(op));
-    op.out = out;
-    op.chunk_counter = (unsigned long long *)c->nl_count.p;
-    op.total = (unsigned long long *)c->nl_count.p + 1;
-    op.chunk_used = (int *)c->ch_used.p;
-    op.max_chunks = nchunks;
-    if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, grid))) return s;
-    const unsigned long long *tot = (const unsigned long long *)c->nl_count.p + 1;
-    LAUNCH(c, "k_nl_holes", hipLaunchKernelGGL(k_nl_holes, dim3(blocks(nchunks + 1, 256)), dim3(256), 0, c->stream,
-                                               (long long)nchunks, (const int *)c->ch_used.p, tot, (long long)cap,
-                                               (int *)c->ch_holes.p, (int *)c->ch_movers.p));
-    if ((s = scan_int(c, (const int *)c->ch_holes.p, (int *)c->ch_hstart.p, (int)nchunks + 1))) return s;
-    if ((s = scan_int(c, (const int *)c->ch_movers.p, (int *)c->ch_mstart.p, (int)nchunks + 1))) return s;
-    LAUNCH(c, "k_nl_fill", hipLaunchKernelGGL((k_nl_fill<T>), dim3(std::max(1, std::min(2048, c->num_cus * 8))), dim3(256), 0,
-                                              c->stream, (long long)nchunks, (const int *)c->ch_used.p,
-                                              (const int *)c->ch_hstart.p, (const int *)c->ch_mstart.p, tot, out));
+    // Spill slots beyond the capacity: every wave holds at most one partial and one unused
+    // prefetched chunk, and the interleaved counters can run ahead of each other by a few
+    // standard deviations of sqrt(active) chunks each.  A pair beyond the spill raises the
+    // overflow word; the sweep is then repeated with twice the spill.
+    int64_t scap = ((int64_t)3 * active + 4 * NL_COUNTERS * ((int64_t)std::sqrt((double)active) + 1) + NL_COUNTERS) *
+                   NL_CHUNK;
+    const int nctr = NL_COUNTERS * NL_COUNTER_STRIDE;
+    HIPCHK(c, ensure(c->nl_count, sizeof(unsigned long long) * (nctr + 2)));
+    unsigned long long *ctr = (unsigned long long *)c->nl_count.p;
+    unsigned long long *tot = ctr + nctr;
+    unsigned int *ovf = (unsigned int *)(ctr + nctr + 1);
     unsigned long long cnt = 0;
-    HIPCHK(c, hipMemcpyAsync(&cnt, tot, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
-    if ((s = sync(c))) return s;
+    int64_t nchunks = 0;
+    NLOut<T> out;
+    for (int attempt = 0;; ++attempt) {
+        nchunks = (cap + scap + NL_CHUNK - 1) / NL_CHUNK + 1;
+        HIPCHK(c, ensure(c->sp_i, sizeof(int32_t) * scap));
+        HIPCHK(c, ensure(c->sp_j, sizeof(int32_t) * scap));
+        HIPCHK(c, ensure(c->sp_d, sizeof(T) * scap));
+        for (DBuf *b : {&c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart})
+            HIPCHK(c, ensure(*b, sizeof(int) * (nchunks + 1)));
+        if ((s = fill_int(c, c->nl_count.p, 2 * (nctr + 2), 0))) return s;
+        if ((s = fill_int(c, c->ch_used.p, nchunks + 1, 0))) return s;
+        out.oi = di; out.oj = dj; out.od = dd;
+        out.si = (int32_t *)c->sp_i.p; out.sj = (int32_t *)c->sp_j.p; out.sd = (T *)c->sp_d.p;
+        out.cap = cap;
+        out.scap = scap;
+        out.overflow = ovf;
+        OpNeighborList<T> op;
+        std::memset((void *)&op, 0, sizeof(op));
+        op.out = out;
+        op.chunk_counter = ctr;
+        op.total = tot;
+        op.chunk_used = (int *)c->ch_used.p;
+        op.max_chunks = nchunks;
+        if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, grid))) return s;
+        // compaction (a no-op on the device when the pairs exceed the capacity), then one
+        // host synchronisation for the count and the overflow word
+        LAUNCH(c, "k_nl_holes", hipLaunchKernelGGL(k_nl_holes, dim3(blocks(nchunks + 1, 256)), dim3(256), 0, c->stream,
+                                                   (long long)nchunks, (const int *)c->ch_used.p, tot, (long long)cap,
+                                                   (int *)c->ch_holes.p, (int *)c->ch_movers.p));
+        if ((s = scan_int(c, (const int *)c->ch_holes.p, (int *)c->ch_hstart.p, (int)nchunks + 1))) return s;
+        if ((s = scan_int(c, (const int *)c->ch_movers.p, (int *)c->ch_mstart.p, (int)nchunks + 1))) return s;
+        LAUNCH(c, "k_nl_fill", hipLaunchKernelGGL((k_nl_fill<T>), dim3(std::max(1, std::min(2048, c->num_cus * 8))), dim3(256),
+                                                  0, c->stream, (long long)nchunks, (const int *)c->ch_used.p,
+                                                  (const int *)c->ch_hstart.p, (const int *)c->ch_mstart.p, tot, out));
+        unsigned int h_ovf = 0;
+        HIPCHK(c, hipMemcpyAsync(&cnt, tot, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&h_ovf, ovf, sizeof(h_ovf), hipMemcpyDeviceToHost, c->stream));
+        if ((s = sync(c))) return s;
+        if (!h_ovf || (int64_t)cnt > cap) break;
+        if (attempt >= 3) return fail(c, CLM_ERR_HIP, "neighbor list spill area overflow");
+        scap *= 2;
+    }
     *npairs = (int64_t)cnt;
     if (!on_device && cnt > 0 && (int64_t)cnt <= cap) {
         const int64_t m = (int64_t)cnt;
@@ -1011,7 +1131,7 @@ clm_status clm_destroy(clm_ctx *c) {
                     &c->scan_tmp, &c->nl_i, &c->nl_j, &c->nl_d, &c->nl_count, &c->outbuf, &c->acc,
                     &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b, &c->sp_i, &c->sp_j, &c->sp_d,
                     &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart, &c->sys_desc,
-                    &c->sys_of, &c->rowoff, &c->E_raw, &c->rstart, &c->real_pos};
+                    &c->sys_of, &c->rowoff, &c->E_raw, &c->rstart, &c->real_pos, &c->cnt64, &c->maxcnt, &c->order};
     for (DBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

@@ -153,31 +153,185 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, const SysDesc<T> *__
     if (flags) atomicOr(gflags, flags);
 }
 
+// Slot claim shared by the lanes that add to the same counter: one atomic per distinct
+// counter in the wave, issued back to back (each group's leader keeps the returned base)
+// and only then handed to the group's lanes, so the atomics' latencies overlap.
+__device__ inline uint32_t grouped_add(unsigned long long *ctr, int c, bool act, unsigned long long inc) {
+    unsigned long long base = 0;
+    int my_leader = 0, my_rank = 0;
+    uint64_t todo = ballot64(act);
+    while (todo) {
+        const int leader = (int)__builtin_ctzll(todo);
+        const int cl = __builtin_amdgcn_readlane(c, leader);
+        const bool mine = act && c == cl;
+        const uint64_t grp = ballot64(mine);
+        if (lane_id() == leader) base = atomicAdd(&ctr[cl], inc * (unsigned long long)__popcll(grp));
+        if (mine) {
+            my_leader = leader;
+            my_rank = rank_in(grp);
+        }
+        todo &= ~grp;
+    }
+    return (uint32_t)__shfl((int)(uint32_t)base, my_leader, 64) + (uint32_t)my_rank;
+}
+
+// One particle of the single-pass binning (see k_bin_bucket); b/coff are the particle's
+// system, passed uniformly when every lane of the wave is in the same system so that the
+// box is read through the scalar cache.
+template <typename T, int D>
+__device__ inline uint32_t bin_one(const DevBox<T> &b, int coff, const T *__restrict__ x, int i, bool in,
+                                   const int *__restrict__ types, unsigned long long *__restrict__ cnt64,
+                                   Ent<T> *__restrict__ Eb, int bucket) {
+    const bool periodic = b.celltype != NONPER_FLAG;
+    T xi[3] = {T(0), T(0), T(0)};
+    bool finite = true;
+    if (in) {
+        for (int d = 0; d < D; ++d) {
+            xi[d] = x[(size_t)i * D + d];
+            finite = finite && dev_finite(xi[d]);
+        }
+    }
+    const bool live = in && finite;
+    uint32_t flags = (in && !finite) ? FLAG_NAN : 0u;
+    T p[3] = {T(0), T(0), T(0)};
+    if (live) {
+        if (periodic) {
+            if (D == 3) wrap_rotate3(b, xi, p);
+            else wrap_rotate2(b, xi, p);
+        } else {
+            for (int d = 0; d < D; ++d) p[d] = xi[d];
+        }
+    }
+    const uint32_t tbits = (live && types) ? ((uint32_t)types[i] & 7u) << TYPE_SHIFT : 0u;
+    const int c = live ? coff + cell_of(b, p, true, flags) : 0;
+    {
+        const uint32_t slot = grouped_add(cnt64, c, live, (1ull << 32) | 1ull);
+        if (live) {
+            Ent<T> e;
+            e.x = p[0]; e.y = p[1]; e.z = p[2];
+            e.w = (uint32_t)i | tbits;
+            if (slot < (uint32_t)bucket) Eb[(size_t)c * bucket + slot] = e;
+            else flags |= FLAG_OVERFLOW;
+        }
+    }
+    if (ballot64(live && periodic) != 0) {
+        const int nimg = D == 3 ? 27 : 9;
+        for (int m = 0; m < nimg; ++m) {
+            if (m == (D == 3 ? 13 : 4)) continue;
+            T q[3] = {p[0], p[1], p[2]};
+            for (int d = 0; d < D; ++d) q[d] = p[d] + b.shift[m][d];
+            const bool img = live && periodic && in_box(b, q);
+            if (ballot64(img) == 0) continue;
+            const int cj = img ? coff + cell_of(b, q, false, flags) : 0;
+            const uint32_t slot = grouped_add(cnt64, cj, img, 1ull);
+            if (img) {
+                Ent<T> g;
+                g.x = q[0]; g.y = q[1]; g.z = q[2];
+                g.w = (uint32_t)i | tbits | GHOST_BIT;
+                if (slot < (uint32_t)bucket) Eb[(size_t)cj * bucket + slot] = g;
+                else flags |= FLAG_OVERFLOW;
+            }
+        }
+    }
+    return flags;
+}
+
+// Single-pass binning into fixed-size cell buckets: every real particle and each of its
+// ghost images takes a slot of its cell from the cell's 64-bit counter word (entries in
+// the low half, reals in the high half) and is written at c * bucket + slot.  Particles
+// are visited in the cell order of the previous build (order[], from k_cell_sort), so a
+// wave's particles mostly fall in a few cells and grouped_add claims their slots with a
+// few atomics.  Replaces k_bin_count + scan + k_bin_scatter (two passes and three atomics
+// per real particle) when the bucket size covers the fullest cell; a slot at or beyond
+// the bucket raises FLAG_OVERFLOW and the host repeats the pass with larger buckets.
+template <typename T, int D>
+__global__ void __launch_bounds__(256) k_bin_bucket(const T *__restrict__ x, int n, const int *__restrict__ order,
+                                                    const SysDesc<T> *__restrict__ sys, const int *__restrict__ sys_of,
+                                                    const int *__restrict__ types, unsigned long long *__restrict__ cnt64,
+                                                    Ent<T> *__restrict__ Eb, int bucket, uint32_t *__restrict__ gflags) {
+    // every lane stays in the wave to the end: grouped_add is a wave-wide operation
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = k < n;
+    const int i = in ? (order ? order[k] : k) : 0;
+    const int si = (in && sys_of) ? sys_of[i] : 0;
+    const int s0 = uniform_int(si);
+    uint32_t flags;
+    if (ballot64(si != s0) == 0) flags = bin_one<T, D>(sys[s0].box, sys[s0].cell_off, x, i, in, types, cnt64, Eb, bucket);
+    else flags = bin_one<T, D>(sys[si].box, sys[si].cell_off, x, i, in, types, cnt64, Eb, bucket);
+    if (flags) atomicOr(gflags, flags);
+}
+
+// Largest of a block's values into *maxcnt (one atomic per block).
+__device__ inline void block_max_to(int mx, int *maxcnt) {
+    __shared__ int wm[4];
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = wm[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = max(m, wm[w]);
+        if (m > 0) atomicMax(maxcnt, m);
+    }
+}
+
+// Largest cell count (sizes the buckets of the single-pass binning).  Grid-stride,
+// 256 threads per block.
+__global__ void __launch_bounds__(256) k_count_max(int ncells, const int *__restrict__ cnt, int *__restrict__ maxcnt) {
+    int mx = 0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) mx = max(mx, cnt[c]);
+    block_max_to(mx, maxcnt);
+}
+
+// Splits the bucket counter words into entry and real counts (with the trailing zero the
+// scans need) and records the fullest cell.  Grid-stride, 256 threads per block.
+__global__ void __launch_bounds__(256) k_bin_unpack(int ncells, const unsigned long long *__restrict__ cnt64,
+                                                    int *__restrict__ cnt, int *__restrict__ creal,
+                                                    int *__restrict__ maxcnt) {
+    int mx = 0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncells; c += gridDim.x * blockDim.x) {
+        if (c == ncells) {
+            cnt[c] = 0;
+            creal[c] = 0;
+            continue;
+        }
+        const unsigned long long v = cnt64[c];
+        const int ne = (int)(uint32_t)v;
+        cnt[c] = ne;
+        creal[c] = (int)(uint32_t)(v >> 32);
+        mx = max(mx, ne);
+    }
+    block_max_to(mx, maxcnt);
+}
+
 // Orders the entries of every cell by (x, w) -- x in the aligned frame, w breaking ties
 // (unique per entry) -- and records where each real particle landed: real_pos[rstart[c]
-// + k] = position of the k-th real of cell c.  Cells are contiguous intervals of x along a
+// + k] = position of the k-th real of cell c, order[rstart[c] + k] = its particle index
+// (the visiting order of the next binning pass).  Cells are contiguous intervals of x along a
 // row of the grid, so a whole row of cells is then ordered by x (up to rounding at the two
 // cells the border rule moves reals into), which lets the sweep stop a lane's walk along a
 // row as soon as it passes the end of that lane's window.  The order is also independent
 // of the atomic arrival order of the scatter.  One wave per cell; ranks by counting.
+// Ein holds the entries of cell c at cstart[c] (compact) or, when bucket > 0, at c * bucket.
 template <typename T>
 __global__ void __launch_bounds__(64) k_cell_sort(int ncells, const int *__restrict__ cstart,
                                                   const int *__restrict__ rstart, const Ent<T> *__restrict__ Ein,
-                                                  Ent<T> *__restrict__ Eout, int *__restrict__ real_pos) {
+                                                  int bucket, Ent<T> *__restrict__ Eout, int *__restrict__ real_pos,
+                                                  int *__restrict__ order) {
     const int lane = threadIdx.x;
     for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
         const int s0 = cstart[c];
         const int n = cstart[c + 1] - s0;
         if (n == 0) continue;
         const int r0 = rstart[c];
+        const Ent<T> *src = Ein + (bucket > 0 ? (size_t)c * bucket : (size_t)s0);
         for (int k0 = 0; k0 < n; k0 += 64) {
             const int k = k0 + lane;
             const bool mine = k < n;
-            const Ent<T> e = Ein[s0 + (mine ? k : 0)];
+            const Ent<T> e = src[mine ? k : 0];
             const uint32_t we = ent_w(e);
             int rank = 0, rrank = 0;
             for (int m = 0; m < n; ++m) {
-                const Ent<T> f = Ein[s0 + m];
+                const Ent<T> f = src[m];
                 const uint32_t wf = ent_w(f);
                 const bool less = (f.x < e.x) || (f.x == e.x && wf < we);
                 rank += less ? 1 : 0;
@@ -185,7 +339,10 @@ __global__ void __launch_bounds__(64) k_cell_sort(int ncells, const int *__restr
             }
             if (mine) {
                 Eout[s0 + rank] = e;
-                if (!(we & GHOST_BIT)) real_pos[r0 + rrank] = s0 + rank;
+                if (!(we & GHOST_BIT)) {
+                    real_pos[r0 + rrank] = s0 + rank;
+                    order[r0 + rrank] = (int)(we & IDX_MASK);
+                }
             }
         }
     }
@@ -243,6 +400,7 @@ struct SweepArgs {
     int *task_counter;         // [8] per-XCD counters, zeroed before launch
     int nrows;                 // rows over all systems
     int nsys;
+    uint32_t en_bytes;         // bytes of the neighbour entry array (buffer-load range)
 };
 
 // Dynamic task fetch: blocks are dealt to XCDs round-robin (block b runs on XCD b % 8),
@@ -415,6 +573,27 @@ __device__ inline int axis_cell(const DevBox<T> &b, int a, T v, T inv) {
     return (int)f;
 }
 
+// Neighbour entries through a raw buffer resource: 32-bit per-lane byte offsets, and a
+// load past the end of the array returns zeros instead of touching memory (lanes whose
+// window is exhausted keep loading without bounds arithmetic).
+__device__ inline __amdgpu_buffer_rsrc_t entry_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ inline Ent<T> load_entry(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ inline Ent<float> load_entry<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(Ent<float>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+template <>
+__device__ inline Ent<double> load_entry<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    struct Two { __attribute__((ext_vector_type(4))) unsigned a, b; };
+    Two t;
+    t.a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    t.b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, 0);
+    return __builtin_bit_cast(Ent<double>, t);
+}
+
 template <typename T>
 __device__ inline T window_sqrt(T v);
 template <>
@@ -429,9 +608,13 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
     __shared__ typename Op::Shared sh;
     __shared__ int s_task;
     const int lane = threadIdx.x;
-    constexpr int U = 4;  // entries per lane in flight
+#ifndef CLM_SWEEP_U
+#define CLM_SWEEP_U 4
+#endif
+    constexpr int U = CLM_SWEEP_U;  // entries per lane per step (and as many prefetched)
     typename Op::Wave ws;
     op.wave_init(sh, ws);
+    const __amdgpu_buffer_rsrc_t rs = entry_rsrc(En, sa.en_bytes);
     const int ntasks = sa.tstart[sa.nrows];
     int home_xcd = blockIdx.x & 7, probe = 0;
     while (true) {
@@ -500,21 +683,19 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                 int slo = 0, shi = 0;
                 if (MODE == MODE_X_ORTHO && selfrow) { slo = csn[base + cx]; shi = csn[base + cx + 1]; }
                 const int maxlen = wave_max_int(len);
+                const uint32_t vb = (uint32_t)j0 * (uint32_t)sizeof(Ent<T>);
                 Ent<T> cur[U], nxt[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) cur[u] = En[u < len ? j0 + u : j0];
+                for (int u = 0; u < U; ++u) cur[u] = load_entry<T>(rs, vb + (uint32_t)(u * sizeof(Ent<T>)));
                 for (int it = 0; it < maxlen; it += U) {
+                    const uint32_t vn = vb + (uint32_t)((it + U) * sizeof(Ent<T>));
 #pragma unroll
-                    for (int u = 0; u < U; ++u) nxt[u] = En[it + U + u < len ? j0 + it + U + u : j0];
+                    for (int u = 0; u < U; ++u) nxt[u] = load_entry<T>(rs, vn + (uint32_t)(u * sizeof(Ent<T>)));
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int t = it + u;
                         const Ent<T> &q = cur[u];
-                        bool act = t < len;
-                        if (act && q.x > xend) {  // the row is ordered by x: the window is over
-                            len = t;
-                            act = false;
-                        }
+                        const bool act = t < len;
                         const uint32_t wq = ent_w(q);
                         const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
                         T d2 = dx * dx + dy2 * dy2;
@@ -533,6 +714,9 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                         }
                         op.candidate(sh, ws, ls, ok, p, q, d2);
                     }
+                    // The row is ordered by x: once an entry lies past the window's end the
+                    // rest of the lane's range is too (entries in between are still tested).
+                    if (cur[U - 1].x > xend) len = min(len, it + U);
                     if (ballot64(it + U < len) == 0) break;
 #pragma unroll
                     for (int u = 0; u < U; ++u) cur[u] = nxt[u];
@@ -552,11 +736,17 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
 // LDS and written out in coalesced blocks.  Output space is handed out in chunks of CH
 // slots: a wave owns one chunk at a time and requests the next one with a single
 // device-scope atomic issued well before it is needed, so the atomic's latency is hidden
-// and there are ~P/CH atomics instead of one per flush.  Slots at or beyond the caller's
-// capacity go to a library-owned spill area.  The partially used last chunk of every wave
-// leaves holes; k_nl_holes / k_nl_fill move entries from beyond the pair count into them,
-// so the final list is dense (its order is unspecified, as in the reference).
-constexpr int NL_CHUNK = 4096;
+// and there are ~P/CH atomics instead of one per flush.  The atomics are spread over
+// NL_COUNTERS counters on separate cache lines (one address serialises them, ~13 ns
+// each): counter k hands out the chunks k, k + K, k + 2K, ..., and a wave takes its
+// successive requests from successive counters, so the counters advance evenly.  Slots at
+// or beyond the caller's capacity go to a library-owned spill area (a slot past it sets
+// an overflow word).  Partially used chunks and chunks never handed out below the pair
+// count leave holes; k_nl_holes / k_nl_fill move entries from beyond the pair count into
+// them, so the final list is dense (its order is unspecified, as in the reference).
+constexpr int NL_CHUNK = 1024;
+constexpr int NL_COUNTERS = 64;
+constexpr int NL_COUNTER_STRIDE = 16;  // u64 words between counters (128 B)
 
 template <typename T>
 struct NLOut {
@@ -565,6 +755,7 @@ struct NLOut {
     int32_t *si, *sj;
     T *sd;
     long long cap, scap;
+    unsigned int *overflow;  // set when a slot lies beyond the spill area (its pair is dropped)
     __device__ void put(long long s, int i, int j, T d) const {
         const bool u = s < cap;
         const long long q = u ? s : s - cap;
@@ -573,6 +764,8 @@ struct NLOut {
         T *pd = u ? od : sd;
         if (u || q < scap) {
             pi[q] = i; pj[q] = j; pd[q] = d;
+        } else {
+            atomicOr(overflow, 1u);
         }
     }
     __device__ void get(long long s, int &i, int &j, T &d) const {
@@ -595,7 +788,7 @@ template <typename T>
 struct OpNeighborList {
     static constexpr int S = 512;
     static constexpr int CH = NL_CHUNK;
-    static constexpr int PREFETCH = 1536;  // request the next chunk when fewer slots remain
+    static constexpr int PREFETCH = 640;  // request the next chunk when fewer slots remain
     struct Shared {
         int i[S];
         int j[S];
@@ -610,17 +803,25 @@ struct OpNeighborList {
         long long cur;
         unsigned long long nxt_raw;  // lane 0 holds the pending atomic result
         unsigned long long written;
+        int nxt_k;                   // counter of the pending request
+        int req;                     // requests made so far
         bool have_nxt;
     };
     NLOut<T> out;
-    unsigned long long *chunk_counter;
+    unsigned long long *chunk_counter;  // NL_COUNTERS counters, NL_COUNTER_STRIDE words apart
     unsigned long long *total;
     int *chunk_used;
     long long max_chunks;
 
     __device__ void request(Wave &w) const {
-        w.nxt_raw = threadIdx.x == 0 ? atomicAdd(chunk_counter, 1ull) : 0ull;
+        const int k = (int)((blockIdx.x + (unsigned)w.req) % NL_COUNTERS);
+        w.nxt_raw = threadIdx.x == 0 ? atomicAdd(&chunk_counter[k * NL_COUNTER_STRIDE], 1ull) : 0ull;
+        w.nxt_k = k;
+        w.req += 1;
         w.have_nxt = true;
+    }
+    __device__ long long granted(const Wave &w) const {
+        return (long long)uniform_u64(w.nxt_raw) * NL_COUNTERS + w.nxt_k;
     }
     __device__ void wave_init(Shared &, Wave &w) const {
         w.staged = 0;
@@ -628,6 +829,8 @@ struct OpNeighborList {
         w.cur_used = CH;
         w.written = 0;
         w.nxt_raw = 0;
+        w.nxt_k = 0;
+        w.req = 0;
         w.have_nxt = false;  // the first chunk is requested at the first flush
     }
     __device__ void lane_init(Wave &w, Lane &l, const Ent<T> &, bool) const { l.n = w.staged; }
@@ -641,7 +844,7 @@ struct OpNeighborList {
             if (w.cur_used == CH) {
                 retire(w, CH);
                 if (!w.have_nxt) request(w);
-                w.cur = (long long)uniform_u64(w.nxt_raw);
+                w.cur = granted(w);
                 w.cur_used = 0;
                 w.have_nxt = false;
             }
@@ -678,7 +881,7 @@ struct OpNeighborList {
         if (w.staged > 0) flush(sh, w, w.staged);
         if (w.cur >= 0) retire(w, w.cur_used);
         if (w.have_nxt) {
-            const long long nx = (long long)uniform_u64(w.nxt_raw);
+            const long long nx = granted(w);
             if (nx < max_chunks && threadIdx.x == 0) chunk_used[nx] = 0;
         }
         if (threadIdx.x == 0 && w.written) atomicAdd(total, w.written);
@@ -752,6 +955,7 @@ struct OpSoftCore {
     struct Shared {
         double gde[8];
         double gds[8];
+        T eps[64], sig2[64], sig[64];  // pair tables, read with a per-lane index
     };
     struct Lane {
         T fx, fy, fz;
@@ -769,6 +973,9 @@ struct OpSoftCore {
     struct Wave {};
     __device__ void wave_init(Shared &sh, Wave &) const {
         if (threadIdx.x < 8) { sh.gde[threadIdx.x] = 0; sh.gds[threadIdx.x] = 0; }
+        sh.eps[threadIdx.x] = prm.eps[threadIdx.x];
+        sh.sig2[threadIdx.x] = prm.sig2[threadIdx.x];
+        sh.sig[threadIdx.x] = prm.sig[threadIdx.x];
         __syncthreads();
     }
     __device__ void lane_init(Wave &, Lane &l, const Ent<T> &p, bool) const {
@@ -777,13 +984,13 @@ struct OpSoftCore {
         l.aeps = l.asig = T(0);
         l.ti = prm.ntypes ? (int)((ent_w(p) >> TYPE_SHIFT) & 7u) : 0;
     }
-    __device__ void candidate(Shared &, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
+    __device__ void candidate(Shared &sh, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         if (!ok) return;
         int t = 0;
         if (prm.ntypes) t = l.ti * 8 + (int)((ent_w(q) >> TYPE_SHIFT) & 7u);
-        const T e = prm.eps[t];
+        const T e = sh.eps[t];
         const T inv = fast_rcp(d2 + prm.a);
-        const T s = prm.sig2[t] * inv;
+        const T s = sh.sig2[t] * inv;
         const T s3 = s * s * s;
         const T s6 = s3 * s3;
         l.e += e * (s6 - prm.c * s3);
@@ -796,7 +1003,7 @@ struct OpSoftCore {
         }
         if (ADJ) {
             l.aeps += (s6 - prm.c * s3) * e;
-            l.asig += e * w / prm.sig[t];
+            l.asig += e * w / sh.sig[t];
         }
     }
     __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &p, bool valid, const SysDesc<T> &sd,
