@@ -406,12 +406,16 @@ clm_status enqueue_count(clm_ctx *c) {
 
 // Order every cell by x into E and list the real particles' positions (k_cell_sort); the
 // unsorted entries are in E_raw, compact (bucket 0) or in cell buckets.
+// threads: an upper bound of the entry count (the count itself when known on the host, the
+// entry capacity inside a captured graph).
 template <typename T>
-clm_status enqueue_sort(clm_ctx *c, int bucket) {
-    if (c->ncells <= 0) return CLM_OK;
-    const int grid = std::max(1, std::min(c->ncells, c->num_cus * 32));
-    LAUNCH(c, "k_cell_sort", hipLaunchKernelGGL((k_cell_sort<T>), dim3(grid), dim3(64), 0, c->stream, c->ncells,
-                                                (const int *)c->cstart.p, (const int *)c->rstart.p,
+clm_status enqueue_sort(clm_ctx *c, int bucket, int64_t threads) {
+    if (c->ncells <= 0 || threads <= 0) {
+        c->order_valid = false;
+        return CLM_OK;
+    }
+    LAUNCH(c, "k_cell_sort", hipLaunchKernelGGL((k_cell_sort<T>), dim3(blocks(threads, 256)), dim3(256), 0, c->stream,
+                                                c->ncells, (const int *)c->cstart.p, (const int *)c->rstart.p,
                                                 (const Ent<T> *)c->E_raw.p, bucket, (Ent<T> *)c->E.p,
                                                 (int *)c->real_pos.p, (int *)c->order.p));
     c->order_valid = true;
@@ -444,7 +448,7 @@ clm_status enqueue_scatter(clm_ctx *c) {
                                                           (int *)c->cur_ghost.p, (Ent<T> *)c->E_raw.p, c->e_cap,
                                                           (uint32_t *)c->flags.p));
     }
-    return enqueue_sort<T>(c, 0);
+    return enqueue_sort<T>(c, 0, c->e_cap);
 }
 
 clm_status check_flags(clm_ctx *c, uint32_t fl) {
@@ -517,7 +521,7 @@ clm_status bin_bucketed(clm_ctx *c, bool &done) {
         }
         HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
         c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
-        if ((s = enqueue_sort<T>(c, c->bucket))) return s;
+        if ((s = enqueue_sort<T>(c, c->bucket, total))) return s;
         done = true;
         return CLM_OK;
     }

@@ -57,6 +57,16 @@ struct SysDesc {
     int n;
 };
 
+// Largest k in [lo, hi) with a[k] <= v (a non-decreasing, a[lo] <= v).
+__device__ inline int last_le(const int *__restrict__ a, int lo, int hi, int v) {
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 template <typename T, int D>
 __global__ void __launch_bounds__(256) k_bin_count(const T *__restrict__ x, int n, const SysDesc<T> *__restrict__ sys,
                                                    const int *__restrict__ sys_of, Ent<T> *__restrict__ xw,
@@ -153,25 +163,27 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, const SysDesc<T> *__
     if (flags) atomicOr(gflags, flags);
 }
 
-// Slot claim shared by the lanes that add to the same counter: one atomic per distinct
-// counter in the wave, issued back to back (each group's leader keeps the returned base)
-// and only then handed to the group's lanes, so the atomics' latencies overlap.
+// Slot claim shared by the lanes that add to the same counter: the lanes are grouped by
+// counter (one ballot per distinct counter), then every group's first lane adds the
+// group's total in one wave-wide atomic instruction (all groups' atomics in flight
+// together) and hands the returned base to its group.
 __device__ inline uint32_t grouped_add(unsigned long long *ctr, int c, bool act, unsigned long long inc) {
-    unsigned long long base = 0;
-    int my_leader = 0, my_rank = 0;
+    int my_leader = 0, my_rank = 0, my_size = 0;
     uint64_t todo = ballot64(act);
     while (todo) {
         const int leader = (int)__builtin_ctzll(todo);
         const int cl = __builtin_amdgcn_readlane(c, leader);
         const bool mine = act && c == cl;
         const uint64_t grp = ballot64(mine);
-        if (lane_id() == leader) base = atomicAdd(&ctr[cl], inc * (unsigned long long)__popcll(grp));
         if (mine) {
             my_leader = leader;
             my_rank = rank_in(grp);
+            my_size = __popcll(grp);
         }
         todo &= ~grp;
     }
+    unsigned long long base = 0;
+    if (act && my_rank == 0) base = atomicAdd(&ctr[c], inc * (unsigned long long)my_size);
     return (uint32_t)__shfl((int)(uint32_t)base, my_leader, 64) + (uint32_t)my_rank;
 }
 
@@ -310,41 +322,37 @@ __global__ void __launch_bounds__(256) k_bin_unpack(int ncells, const unsigned l
 // row of the grid, so a whole row of cells is then ordered by x (up to rounding at the two
 // cells the border rule moves reals into), which lets the sweep stop a lane's walk along a
 // row as soon as it passes the end of that lane's window.  The order is also independent
-// of the atomic arrival order of the scatter.  One wave per cell; ranks by counting.
+// of the atomic arrival order of the scatter.  One thread per entry: it finds its cell by
+// bisection of cstart and its rank by counting the smaller entries of the cell.
 // Ein holds the entries of cell c at cstart[c] (compact) or, when bucket > 0, at c * bucket.
+// The grid covers at least cstart[ncells] threads (the entry count, read on the device so
+// that a captured graph sorts whatever the count of its replay is, within the capacity).
 template <typename T>
-__global__ void __launch_bounds__(64) k_cell_sort(int ncells, const int *__restrict__ cstart,
-                                                  const int *__restrict__ rstart, const Ent<T> *__restrict__ Ein,
-                                                  int bucket, Ent<T> *__restrict__ Eout, int *__restrict__ real_pos,
-                                                  int *__restrict__ order) {
-    const int lane = threadIdx.x;
-    for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
-        const int s0 = cstart[c];
-        const int n = cstart[c + 1] - s0;
-        if (n == 0) continue;
+__global__ void __launch_bounds__(256) k_cell_sort(int ncells, const int *__restrict__ cstart,
+                                                   const int *__restrict__ rstart, const Ent<T> *__restrict__ Ein,
+                                                   int bucket, Ent<T> *__restrict__ Eout, int *__restrict__ real_pos,
+                                                   int *__restrict__ order) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cstart[ncells]) return;
+    const int c = last_le(cstart, 0, ncells, k);  // the last cell starting at or before k: never empty
+    const int s0 = cstart[c];
+    const int n = cstart[c + 1] - s0;
+    const Ent<T> *src = Ein + (bucket > 0 ? (size_t)c * bucket : (size_t)s0);
+    const Ent<T> e = src[k - s0];
+    const uint32_t we = ent_w(e);
+    int rank = 0, rrank = 0;
+    for (int m = 0; m < n; ++m) {
+        const Ent<T> f = src[m];
+        const uint32_t wf = ent_w(f);
+        const bool less = (f.x < e.x) || (f.x == e.x && wf < we);
+        rank += less ? 1 : 0;
+        rrank += (less && !(wf & GHOST_BIT)) ? 1 : 0;
+    }
+    Eout[s0 + rank] = e;
+    if (!(we & GHOST_BIT)) {
         const int r0 = rstart[c];
-        const Ent<T> *src = Ein + (bucket > 0 ? (size_t)c * bucket : (size_t)s0);
-        for (int k0 = 0; k0 < n; k0 += 64) {
-            const int k = k0 + lane;
-            const bool mine = k < n;
-            const Ent<T> e = src[mine ? k : 0];
-            const uint32_t we = ent_w(e);
-            int rank = 0, rrank = 0;
-            for (int m = 0; m < n; ++m) {
-                const Ent<T> f = src[m];
-                const uint32_t wf = ent_w(f);
-                const bool less = (f.x < e.x) || (f.x == e.x && wf < we);
-                rank += less ? 1 : 0;
-                rrank += (less && !(wf & GHOST_BIT)) ? 1 : 0;
-            }
-            if (mine) {
-                Eout[s0 + rank] = e;
-                if (!(we & GHOST_BIT)) {
-                    real_pos[r0 + rrank] = s0 + rank;
-                    order[r0 + rrank] = (int)(we & IDX_MASK);
-                }
-            }
-        }
+        real_pos[r0 + rrank] = s0 + rank;
+        order[r0 + rrank] = (int)(we & IDX_MASK);
     }
 }
 
@@ -358,15 +366,6 @@ __global__ void k_home_counts(int ncells, int all_entries, const int *__restrict
     hc[c] = all_entries ? (nr > 0 ? cstart[c + 1] - cstart[c] : 0) : nr;
 }
 
-// Largest k in [lo, hi) with a[k] <= v (a non-decreasing, a[lo] <= v).
-__device__ inline int last_le(const int *__restrict__ a, int lo, int hi, int v) {
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (a[mid] <= v) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 template <typename T>
 __global__ void k_row_tasks(int nrows, int nsys, const SysDesc<T> *__restrict__ sys, const int *__restrict__ rowoff,
