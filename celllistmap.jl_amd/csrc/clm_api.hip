@@ -533,7 +533,7 @@ template <typename T>
 clm_status update_cell_list_t(clm_ctx *c) {
     if (!c->has_box) return fail(c, CLM_ERR_STATE, "no box set");
     if (!c->has_pos) return fail(c, CLM_ERR_STATE, "no positions set");
-    if (c->n >= (int64_t)IDX_MASK) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
+    if (c->n >= (int64_t)IDX_LIMIT) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
     clm_status s;
     if (c->celltype == CLM_NONPERIODIC && !c->root) {  // a sub-context gets its box from the owner
         if ((s = nonperiodic_box<T>(c))) return s;
@@ -1248,7 +1248,7 @@ clm_status clm_set_systems(clm_ctx *c, int nsys, const int64_t *n, const double 
         }
         total += n[k];
     }
-    if (total >= (int64_t)IDX_MASK) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
+    if (total >= (int64_t)IDX_LIMIT) return fail(c, CLM_ERR_ARG, "too many particles (limit 2^28-1)");
     std::vector<int> sof(std::max<int64_t>(total, 1));
     for (int k = 0, o = 0; k < nsys; ++k)
         for (int64_t q = 0; q < n[k]; ++q) sof[o++] = k;

@@ -18,9 +18,15 @@
 
 namespace clm {
 
-constexpr uint32_t IDX_MASK = (1u << 28) - 1;
-constexpr uint32_t TYPE_SHIFT = 28;
-constexpr uint32_t GHOST_BIT = 1u << 31;
+// Entry word: particle index << 4 | type << 1 | ghost.  With the index in the high bits,
+// comparing whole words orders entries of different particles by index.
+constexpr uint32_t IDX_LIMIT = 1u << 28;
+__host__ __device__ inline uint32_t w_make(uint32_t idx, uint32_t type, bool ghost) {
+    return (idx << 4) | ((type & 7u) << 1) | (ghost ? 1u : 0u);
+}
+__host__ __device__ inline uint32_t w_idx(uint32_t w) { return w >> 4; }
+__host__ __device__ inline uint32_t w_type(uint32_t w) { return (w >> 1) & 7u; }
+__host__ __device__ inline bool w_ghost(uint32_t w) { return (w & 1u) != 0; }
 
 template <typename T> struct Ent;
 template <> struct alignas(16) Ent<float> { float x, y, z; uint32_t w; };

@@ -135,8 +135,8 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, const SysDesc<T> *__
     const int coff = sys[si].cell_off;
     uint32_t flags = 0;
     Ent<T> e = xw[i];
-    uint32_t tbits = types ? ((uint32_t)types[i] & 7u) << TYPE_SHIFT : 0u;
-    e.w = (uint32_t)i | tbits;
+    const uint32_t ty = types ? (uint32_t)types[i] : 0u;
+    e.w = w_make((uint32_t)i, ty, false);
     int slot = atomicAdd(&cur_real[c], 1);
     int64_t pos = (int64_t)cstart[c] + slot;
     if (pos < ecap) E[pos] = e;
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int n, const SysDesc<T> *__
             int64_t pj = (int64_t)cstart[cj] + creal[cj] + s2;
             Ent<T> g;
             g.x = q[0]; g.y = q[1]; g.z = q[2];
-            g.w = (uint32_t)i | tbits | GHOST_BIT;
+            g.w = w_make((uint32_t)i, ty, true);
             if (pj < ecap) E[pj] = g;
             else flags |= FLAG_OVERFLOW;
         }
@@ -214,14 +214,14 @@ __device__ inline uint32_t bin_one(const DevBox<T> &b, int coff, const T *__rest
             for (int d = 0; d < D; ++d) p[d] = xi[d];
         }
     }
-    const uint32_t tbits = (live && types) ? ((uint32_t)types[i] & 7u) << TYPE_SHIFT : 0u;
+    const uint32_t ty = (live && types) ? (uint32_t)types[i] : 0u;
     const int c = live ? coff + cell_of(b, p, true, flags) : 0;
     {
         const uint32_t slot = grouped_add(cnt64, c, live, (1ull << 32) | 1ull);
         if (live) {
             Ent<T> e;
             e.x = p[0]; e.y = p[1]; e.z = p[2];
-            e.w = (uint32_t)i | tbits;
+            e.w = w_make((uint32_t)i, ty, false);
             if (slot < (uint32_t)bucket) Eb[(size_t)c * bucket + slot] = e;
             else flags |= FLAG_OVERFLOW;
         }
@@ -239,7 +239,7 @@ __device__ inline uint32_t bin_one(const DevBox<T> &b, int coff, const T *__rest
             if (img) {
                 Ent<T> g;
                 g.x = q[0]; g.y = q[1]; g.z = q[2];
-                g.w = (uint32_t)i | tbits | GHOST_BIT;
+                g.w = w_make((uint32_t)i, ty, true);
                 if (slot < (uint32_t)bucket) Eb[(size_t)cj * bucket + slot] = g;
                 else flags |= FLAG_OVERFLOW;
             }
@@ -346,13 +346,13 @@ __global__ void __launch_bounds__(256) k_cell_sort(int ncells, const int *__rest
         const uint32_t wf = ent_w(f);
         const bool less = (f.x < e.x) || (f.x == e.x && wf < we);
         rank += less ? 1 : 0;
-        rrank += (less && !(wf & GHOST_BIT)) ? 1 : 0;
+        rrank += (less && !w_ghost(wf)) ? 1 : 0;
     }
     Eout[s0 + rank] = e;
-    if (!(we & GHOST_BIT)) {
+    if (!w_ghost(we)) {
         const int r0 = rstart[c];
         real_pos[r0 + rrank] = s0 + rank;
-        order[r0 + rrank] = (int)(we & IDX_MASK);
+        order[r0 + rrank] = (int)w_idx(we);
     }
 }
 
@@ -465,8 +465,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
         const int nvalid = min(64, he - hb);
         const int cxmin = uniform_int(cx);
         const uint32_t wp = ent_w(p);
-        const uint32_t ip = wp & IDX_MASK;
-        const bool preal = !(wp & GHOST_BIT);
+        const bool preal = !w_ghost(wp);
         typename Op::Lane ls;
         op.lane_init(ws, ls, p, valid);
         const int dzr = (D == 3) ? L : 0;
@@ -505,16 +504,16 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                             if (D == 3) d2 = d2 + dz2 * dz2;
                             bool ok = d2 <= rc2;
                             if (MODE == MODE_A_ORTHO) {
-                                ok = ok && (j >= lo) && (j < hi) && (preal || !(wq & GHOST_BIT));
+                                ok = ok && (j >= lo) && (j < hi) && (preal || !w_ghost(wq));
                             } else if (MODE == MODE_A_TRIC) {
-                                ok = ok && (j >= lo) && (j < hi) && (ip < (wq & IDX_MASK));
+                                ok = ok && (j >= lo) && (j < hi) && (wp < wq);
                             } else if (MODE == MODE_X_ORTHO) {
                                 const bool same = (j >= slo) && (j < shi);
-                                ok = ok && (j >= lo) && (j < hi) && (preal || (!same && !(wq & GHOST_BIT)));
+                                ok = ok && (j >= lo) && (j < hi) && (preal || (!same && !w_ghost(wq)));
                             } else if (MODE == MODE_X_REAL) {
                                 ok = ok && (j >= lo) && (j < hi);
                             } else {
-                                ok = ok && valid && (ip != (wq & IDX_MASK));
+                                ok = ok && valid && (wp != wq);
                             }
                             op.candidate(sh, ws, ls, ok, p, q, d2);
                         }
@@ -642,8 +641,7 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
         const int cx = c - rowc0;
         const int cy = lr % ncy, cz = lr / ncy;
         const uint32_t wp = ent_w(p);
-        const uint32_t ip = wp & IDX_MASK;
-        const bool preal = !(wp & GHOST_BIT);
+        const bool preal = !w_ghost(wp);
         const T mx = T(1e-3) * bx.cs[0], my = T(1e-3) * bx.cs[1], mz = T(1e-3) * bx.cs[2];
         const T invx = T(1) / bx.cs[0];
         typename Op::Lane ls;
@@ -683,17 +681,15 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                 if (MODE == MODE_X_ORTHO && selfrow) { slo = csn[base + cx]; shi = csn[base + cx + 1]; }
                 const int maxlen = wave_max_int(len);
                 const uint32_t vb = (uint32_t)j0 * (uint32_t)sizeof(Ent<T>);
-                Ent<T> cur[U], nxt[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) cur[u] = load_entry<T>(rs, vb + (uint32_t)(u * sizeof(Ent<T>)));
-                for (int it = 0; it < maxlen; it += U) {
-                    const uint32_t vn = vb + (uint32_t)((it + U) * sizeof(Ent<T>));
-#pragma unroll
-                    for (int u = 0; u < U; ++u) nxt[u] = load_entry<T>(rs, vn + (uint32_t)(u * sizeof(Ent<T>)));
+                // One group of U entries: test them, then stop the lane at the window's end
+                // (the row is ordered by x: once an entry lies past it, the rest of the
+                // lane's range does too; entries in between are still tested).  Returns
+                // whether any lane has entries left.
+                auto group = [&](const Ent<T> *g, int it) -> bool {
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int t = it + u;
-                        const Ent<T> &q = cur[u];
+                        const Ent<T> &q = g[u];
                         const bool act = t < len;
                         const uint32_t wq = ent_w(q);
                         const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
@@ -701,24 +697,35 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                         if (D == 3) d2 = d2 + dz2 * dz2;
                         bool ok = act && d2 <= rc2;
                         if (MODE == MODE_A_ORTHO) {
-                            ok = ok && (preal || !(wq & GHOST_BIT));
+                            ok = ok && (preal || !w_ghost(wq));
                         } else if (MODE == MODE_A_TRIC) {
-                            ok = ok && (ip < (wq & IDX_MASK));
+                            ok = ok && (wp < wq);
                         } else if (MODE == MODE_X_ORTHO) {
                             const int j = j0 + t;
                             const bool same = (j >= slo) && (j < shi);
-                            ok = ok && (preal || (!same && !(wq & GHOST_BIT)));
+                            ok = ok && (preal || (!same && !w_ghost(wq)));
                         } else if (MODE == MODE_B) {
-                            ok = ok && (ip != (wq & IDX_MASK));
+                            ok = ok && (wp != wq);
                         }
                         op.candidate(sh, ws, ls, ok, p, q, d2);
                     }
-                    // The row is ordered by x: once an entry lies past the window's end the
-                    // rest of the lane's range is too (entries in between are still tested).
-                    if (cur[U - 1].x > xend) len = min(len, it + U);
-                    if (ballot64(it + U < len) == 0) break;
+                    if (g[U - 1].x > xend) len = min(len, it + U);
+                    return ballot64(it + U < len) != 0;
+                };
+                auto load = [&](Ent<T> *g, int it) {
+                    const uint32_t v = vb + (uint32_t)(it * sizeof(Ent<T>));
 #pragma unroll
-                    for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+                    for (int u = 0; u < U; ++u) g[u] = load_entry<T>(rs, v + (uint32_t)(u * sizeof(Ent<T>)));
+                };
+                // two buffers used in turn (no register copies): while one group is tested
+                // the next one is in flight
+                Ent<T> ga[U], gb[U];
+                load(ga, 0);
+                for (int it = 0; it < maxlen; it += 2 * U) {
+                    load(gb, it + U);
+                    if (!group(ga, it)) break;
+                    load(ga, it + 2 * U);
+                    if (!group(gb, it + U)) break;
                 }
             }
         }
@@ -743,7 +750,7 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
 // an overflow word).  Partially used chunks and chunks never handed out below the pair
 // count leave holes; k_nl_holes / k_nl_fill move entries from beyond the pair count into
 // them, so the final list is dense (its order is unspecified, as in the reference).
-constexpr int NL_CHUNK = 1024;
+constexpr int NL_CHUNK = 512;
 constexpr int NL_COUNTERS = 64;
 constexpr int NL_COUNTER_STRIDE = 16;  // u64 words between counters (128 B)
 
@@ -787,7 +794,7 @@ template <typename T>
 struct OpNeighborList {
     static constexpr int S = 512;
     static constexpr int CH = NL_CHUNK;
-    static constexpr int PREFETCH = 640;  // request the next chunk when fewer slots remain
+    static constexpr int PREFETCH = 384;  // request the next chunk when fewer slots remain
     struct Shared {
         int i[S];
         int j[S];
@@ -860,11 +867,10 @@ struct OpNeighborList {
     }
     __device__ void candidate(Shared &sh, Wave &w, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         const uint64_t m = ballot64(ok);
-        if (m == 0) return;
         if (ok) {
             const int k = l.n + rank_in(m);
-            sh.i[k] = (int)(ent_w(p) & IDX_MASK);
-            sh.j[k] = (int)(ent_w(q) & IDX_MASK);
+            sh.i[k] = (int)w_idx(ent_w(p));
+            sh.j[k] = (int)w_idx(ent_w(q));
             sh.d2[k] = d2;
         }
         l.n += __popcll(m);
@@ -981,12 +987,12 @@ struct OpSoftCore {
         l.fx = l.fy = l.fz = T(0);
         l.e = T(0);
         l.aeps = l.asig = T(0);
-        l.ti = prm.ntypes ? (int)((ent_w(p) >> TYPE_SHIFT) & 7u) : 0;
+        l.ti = prm.ntypes ? (int)w_type(ent_w(p)) : 0;
     }
     __device__ void candidate(Shared &sh, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         if (!ok) return;
         int t = 0;
-        if (prm.ntypes) t = l.ti * 8 + (int)((ent_w(q) >> TYPE_SHIFT) & 7u);
+        if (prm.ntypes) t = l.ti * 8 + (int)w_type(ent_w(q));
         const T e = sh.eps[t];
         const T inv = fast_rcp(d2 + prm.a);
         const T s = sh.sig2[t] * inv;
@@ -1011,7 +1017,7 @@ struct OpSoftCore {
         ev = wave_sum(ev);
         if (threadIdx.x == 0) atomicAdd(&energy[si], ev);
         if (valid) {
-            const uint32_t i = ent_w(p) & IDX_MASK;
+            const uint32_t i = w_idx(ent_w(p));
             if (FORCES) {
                 T f[3] = {l.fx, l.fy, l.fz};
                 if (sd.box.celltype == 1) {
@@ -1073,7 +1079,7 @@ struct OpHistogram {
     __device__ void lane_init(Wave &, Lane &l, const Ent<T> &p, bool valid) const {
         l.vx = l.vy = l.vz = T(0);
         if (vel && valid) {
-            const uint32_t i = ent_w(p) & IDX_MASK;
+            const uint32_t i = w_idx(ent_w(p));
             T v[3] = {T(0), T(0), T(0)}, o[3];
             for (int k = 0; k < dim; ++k) v[k] = vel[(size_t)i * dim + k];
             rotv(v, o);
@@ -1087,7 +1093,7 @@ struct OpHistogram {
         int b = bf > T(nbins - 1) ? nbins - 1 : (int)bf;
         atomicAdd(&sh.cnt[b], 1u);
         if (vel) {
-            const uint32_t j = ent_w(q) & IDX_MASK;
+            const uint32_t j = w_idx(ent_w(q));
             T v[3] = {T(0), T(0), T(0)}, o[3];
             for (int k = 0; k < dim; ++k) v[k] = vel_nb[(size_t)j * dim + k];
             rotv(v, o);
@@ -1135,7 +1141,7 @@ struct OpMinDist {
     }
     __device__ void candidate(Shared &, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
         if (!ok) return;
-        int i = (int)(ent_w(p) & IDX_MASK), j = (int)(ent_w(q) & IDX_MASK);
+        int i = (int)w_idx(ent_w(p)), j = (int)w_idx(ent_w(q));
         int a = (cross || i < j) ? i : j, b = (cross || i < j) ? j : i;
         if (better(d2, a, b, l.d2, l.a, l.b)) { l.d2 = d2; l.a = a; l.b = b; }
     }
