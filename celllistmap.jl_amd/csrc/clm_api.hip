@@ -191,6 +191,24 @@ clm_status fill_int(clm_ctx *c, void *p, int64_t n, int v) {
     return CLM_OK;
 }
 
+clm_status fill_zero(clm_ctx *c, std::initializer_list<std::pair<void *, int64_t>> bufs) {
+    FillList f;
+    std::memset(&f, 0, sizeof(f));
+    int64_t mx = 0;
+    for (auto &b : bufs) {
+        if (b.second <= 0) continue;
+        if (f.count == 8) return fail(c, CLM_ERR_ARG, "fill list too long");
+        f.p[f.count] = (int *)b.first;
+        f.n[f.count] = (int)b.second;
+        mx = std::max(mx, b.second);
+        ++f.count;
+    }
+    if (!f.count) return CLM_OK;
+    LAUNCH(c, "k_fill_int", hipLaunchKernelGGL(k_fill_multi, dim3(std::min<unsigned>(blocks(mx, 256), 1024)), dim3(256), 0,
+                                               c->stream, f));
+    return CLM_OK;
+}
+
 clm_status scan_int(clm_ctx *c, const int *in, int *out, int n) {
     size_t need = 0;
     hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, n, c->stream);
@@ -384,9 +402,7 @@ clm_status enqueue_count(clm_ctx *c) {
     const int64_t n = c->n;
     const int nc1 = c->ncells + 1;
     clm_status s;
-    if ((s = fill_int(c, c->cnt.p, nc1, 0))) return s;
-    if ((s = fill_int(c, c->creal.p, nc1, 0))) return s;
-    if ((s = fill_int(c, c->flags.p, 4, 0))) return s;
+    if ((s = fill_zero(c, {{c->cnt.p, nc1}, {c->creal.p, nc1}, {c->flags.p, 4}}))) return s;
     const SysDesc<T> *sd = (const SysDesc<T> *)c->sys_desc.p;
     if (n > 0) {
         if (c->dim == 3)
@@ -428,8 +444,7 @@ clm_status enqueue_scatter(clm_ctx *c) {
     const int64_t n = c->n;
     const int nc1 = c->ncells + 1;
     clm_status s;
-    if ((s = fill_int(c, c->cur_real.p, nc1, 0))) return s;
-    if ((s = fill_int(c, c->cur_ghost.p, nc1, 0))) return s;
+    if ((s = fill_zero(c, {{c->cur_real.p, nc1}, {c->cur_ghost.p, nc1}}))) return s;
     const SysDesc<T> *sd = (const SysDesc<T> *)c->sys_desc.p;
     if (n > 0) {
         const int *tp = c->has_types ? (const int *)c->types.p : nullptr;
@@ -487,9 +502,8 @@ clm_status bin_bucketed(clm_ctx *c, bool &done) {
         if (c->bucket <= 0) return CLM_OK;
         HIPCHK(c, ensure(c->E_raw, sizeof(Ent<T>) * (size_t)c->ncells * (size_t)c->bucket));
         clm_status s;
-        if ((s = fill_int(c, c->cnt64.p, 2 * c->ncells, 0))) return s;
-        if ((s = fill_int(c, c->flags.p, 4, 0))) return s;
-        if ((s = fill_int(c, c->maxcnt.p, 1, 0))) return s;
+        // status block c->flags: [0] flags, [1] fullest cell, [2] entry total
+        if ((s = fill_zero(c, {{c->cnt64.p, 2 * (int64_t)c->ncells}, {c->flags.p, 4}}))) return s;
         if (n > 0) {
             if (c->dim == 3)
                 LAUNCH(c, "k_bin_bucket", hipLaunchKernelGGL((k_bin_bucket<T, 3>), dim3(blocks(n, 256)), dim3(256), 0, c->stream,
@@ -502,20 +516,18 @@ clm_status bin_bucketed(clm_ctx *c, bool &done) {
                                                              (unsigned long long *)c->cnt64.p, (Ent<T> *)c->E_raw.p,
                                                              c->bucket, (uint32_t *)c->flags.p));
         }
-        LAUNCH(c, "k_bin_unpack", hipLaunchKernelGGL(k_bin_unpack, dim3(std::min<unsigned>(blocks(nc1, 256), 512)), dim3(256), 0,
+        LAUNCH(c, "k_bin_unpack", hipLaunchKernelGGL(k_bin_unpack, dim3(std::min<unsigned>(blocks(nc1, 256), 128)), dim3(256), 0,
                                                      c->stream, c->ncells,
                                                      (const unsigned long long *)c->cnt64.p, (int *)c->cnt.p,
-                                                     (int *)c->creal.p, (int *)c->maxcnt.p));
+                                                     (int *)c->creal.p, (int *)c->flags.p + 1, (int *)c->flags.p + 2));
         if ((s = scan_int(c, (const int *)c->creal.p, (int *)c->rstart.p, nc1))) return s;
         if ((s = scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1))) return s;
-        int total = 0, mx = 0;
-        HIPCHK(c, hipMemcpyAsync(&total, (int *)c->cstart.p + c->ncells, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(&mx, c->maxcnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         if ((s = sync(c))) return s;
-        if ((s = check_flags(c, *c->h_flags))) return s;
+        if ((s = check_flags(c, c->h_flags[0]))) return s;
+        const int mx = (int)c->h_flags[1], total = (int)c->h_flags[2];
         c->n_entries = total;
-        if (*c->h_flags & FLAG_OVERFLOW) {  // a cell outgrew its bucket: larger buckets, again
+        if (c->h_flags[0] & FLAG_OVERFLOW) {  // a cell outgrew its bucket: larger buckets, again
             c->bucket = bucket_size(c, mx, sizeof(Ent<T>));
             continue;
         }
@@ -619,24 +631,31 @@ inline int mode_table(int mode) {
     return mode == MODE_X_ORTHO ? MODE_A_ORTHO : (mode == MODE_X_REAL ? MODE_B : mode);
 }
 
+// Homes per cell, scanned: the real-particle scan of the binning (rstart) when the homes
+// are the real particles, else a scan of all entries of cells holding a real particle.
+const int *home_start(const clm_ctx *c, int mode) {
+    mode = mode_table(mode);
+    return mode_all_entries(mode) ? (const int *)c->hstart[mode].p : (const int *)c->rstart.p;
+}
+
 template <typename T>
 clm_status enqueue_mode(clm_ctx *c, int mode) {
     mode = mode_table(mode);
     const int nc1 = c->ncells + 1;
-    HIPCHK(c, ensure(c->hc, sizeof(int) * nc1));
-    HIPCHK(c, ensure(c->hstart[mode], sizeof(int) * nc1));
     HIPCHK(c, ensure(c->ntask, sizeof(int) * (c->nrows + 1)));
     HIPCHK(c, ensure(c->tstart[mode], sizeof(int) * (c->nrows + 1)));
-    const int all_entries = mode_all_entries(mode) ? 1 : 0;
-    LAUNCH(c, "k_home_counts", hipLaunchKernelGGL(k_home_counts, dim3(blocks(nc1, 256)), dim3(256), 0, c->stream, c->ncells,
-                                                  all_entries, (const int *)c->cstart.p, (const int *)c->creal.p,
-                                                  (int *)c->hc.p));
     clm_status s;
-    if ((s = scan_int(c, (const int *)c->hc.p, (int *)c->hstart[mode].p, nc1))) return s;
+    if (mode_all_entries(mode)) {
+        HIPCHK(c, ensure(c->hc, sizeof(int) * nc1));
+        HIPCHK(c, ensure(c->hstart[mode], sizeof(int) * nc1));
+        LAUNCH(c, "k_home_counts", hipLaunchKernelGGL(k_home_counts, dim3(blocks(nc1, 256)), dim3(256), 0, c->stream,
+                                                      c->ncells, 1, (const int *)c->cstart.p, (const int *)c->creal.p,
+                                                      (int *)c->hc.p));
+        if ((s = scan_int(c, (const int *)c->hc.p, (int *)c->hstart[mode].p, nc1))) return s;
+    }
     LAUNCH(c, "k_row_tasks", hipLaunchKernelGGL((k_row_tasks<T>), dim3(blocks(c->nrows + 1, 256)), dim3(256), 0, c->stream,
                                                 c->nrows, c->nsys, (const SysDesc<T> *)c->sys_desc.p,
-                                                (const int *)c->rowoff.p, (const int *)c->hstart[mode].p,
-                                                (int *)c->ntask.p));
+                                                (const int *)c->rowoff.p, home_start(c, mode), (int *)c->ntask.p));
     return scan_int(c, (const int *)c->ntask.p, (int *)c->tstart[mode].p, c->nrows + 1);
 }
 
@@ -653,7 +672,7 @@ template <typename T>
 SweepArgs<T> sweep_args(clm_ctx *c, int mode) {
     mode = mode_table(mode);
     SweepArgs<T> sa;
-    sa.hstart = (const int *)c->hstart[mode].p;
+    sa.hstart = home_start(c, mode);
     sa.tstart = (const int *)c->tstart[mode].p;
     sa.real_pos = (const int *)c->real_pos.p;
     sa.rowoff = (const int *)c->rowoff.p;
@@ -789,8 +808,7 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         HIPCHK(c, ensure(c->sp_d, sizeof(T) * scap));
         for (DBuf *b : {&c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart})
             HIPCHK(c, ensure(*b, sizeof(int) * (nchunks + 1)));
-        if ((s = fill_int(c, c->nl_count.p, 2 * (nctr + 2), 0))) return s;
-        if ((s = fill_int(c, c->ch_used.p, nchunks + 1, 0))) return s;
+        if ((s = fill_zero(c, {{c->nl_count.p, 2 * (int64_t)(nctr + 2)}, {c->ch_used.p, nchunks + 1}}))) return s;
         out.oi = di; out.oj = dj; out.od = dd;
         out.si = (int32_t *)c->sp_i.p; out.sj = (int32_t *)c->sp_j.p; out.sd = (T *)c->sp_d.p;
         out.cap = cap;
@@ -814,10 +832,11 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         LAUNCH(c, "k_nl_fill", hipLaunchKernelGGL((k_nl_fill<T>), dim3(std::max(1, std::min(2048, c->num_cus * 8))), dim3(256),
                                                   0, c->stream, (long long)nchunks, (const int *)c->ch_used.p,
                                                   (const int *)c->ch_hstart.p, (const int *)c->ch_mstart.p, tot, out));
-        unsigned int h_ovf = 0;
-        HIPCHK(c, hipMemcpyAsync(&cnt, tot, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(&h_ovf, ovf, sizeof(h_ovf), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->h_flags, tot, sizeof(unsigned long long) + sizeof(unsigned int),
+                                 hipMemcpyDeviceToHost, c->stream));  // pair count, then the overflow word
         if ((s = sync(c))) return s;
+        std::memcpy(&cnt, c->h_flags, sizeof(cnt));
+        const unsigned int h_ovf = c->h_flags[2];
         if (!h_ovf || (int64_t)cnt > cap) break;
         if (attempt >= 3) return fail(c, CLM_ERR_HIP, "neighbor list spill area overflow");
         scap *= 2;

@@ -295,11 +295,12 @@ __global__ void __launch_bounds__(256) k_count_max(int ncells, const int *__rest
 }
 
 // Splits the bucket counter words into entry and real counts (with the trailing zero the
-// scans need) and records the fullest cell.  Grid-stride, 256 threads per block.
+// scans need) and records the fullest cell and the entry total.  Grid-stride, 256 threads
+// per block.
 __global__ void __launch_bounds__(256) k_bin_unpack(int ncells, const unsigned long long *__restrict__ cnt64,
                                                     int *__restrict__ cnt, int *__restrict__ creal,
-                                                    int *__restrict__ maxcnt) {
-    int mx = 0;
+                                                    int *__restrict__ maxcnt, int *__restrict__ total) {
+    int mx = 0, sum = 0;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncells; c += gridDim.x * blockDim.x) {
         if (c == ncells) {
             cnt[c] = 0;
@@ -311,8 +312,17 @@ __global__ void __launch_bounds__(256) k_bin_unpack(int ncells, const unsigned l
         cnt[c] = ne;
         creal[c] = (int)(uint32_t)(v >> 32);
         mx = max(mx, ne);
+        sum += ne;
     }
-    block_max_to(mx, maxcnt);
+    __shared__ int ws[4];
+    sum = wave_sum(sum);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = sum;
+    block_max_to(mx, maxcnt);  // synchronises the block
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += ws[w];
+        if (t) atomicAdd(total, t);
+    }
 }
 
 // Orders the entries of every cell by (x, w) -- x in the aligned frame, w breaking ties
@@ -378,6 +388,18 @@ __global__ void k_row_tasks(int nrows, int nsys, const SysDesc<T> *__restrict__ 
     const int c0 = sys[s].cell_off + (r - rowoff[s]) * ncx;
     int h = hstart[c0 + ncx] - hstart[c0];
     ntask[r] = (h + 63) >> 6;
+}
+
+// Zeroes up to 8 int arrays in one launch (grid-stride).
+struct FillList {
+    int *p[8];
+    int n[8];
+    int count;
+};
+__global__ void k_fill_multi(FillList f) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = 0; k < f.count; ++k)
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < f.n[k]; i += stride) f.p[k][i] = 0;
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
