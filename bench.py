@@ -162,7 +162,7 @@ def bench_adjoint(clm, torch, reps=5):
         sig_t.grad = None
         e = clm.softcore_energy(sys_, xt, eps_t, sig_t, types=types, fused=fused)
         e.backward()
-        return float(e)
+        return float(e.detach())
 
     out = {"N": int(x.shape[0]), "pairs": P}
     for fused in (True, False):
