@@ -615,6 +615,17 @@ __device__ inline Ent<double> load_entry<double>(__amdgpu_buffer_rsrc_t r, uint3
 }
 
 template <typename T>
+__device__ inline T load_entry_x(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ inline float load_entry_x<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ inline double load_entry_x<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+template <typename T>
 __device__ inline T window_sqrt(T v);
 template <>
 __device__ inline float window_sqrt<float>(float v) { return __builtin_amdgcn_sqrtf(v); }
@@ -693,6 +704,16 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                     if (ka <= kb) {
                         j0 = csn[base + ka];
                         int j1 = csn[base + kb + 1];
+                        // skip the entries of the first cell that lie before the window
+                        // (the cell is ordered by x): bisection on their x
+                        const T xbeg = p.x - hw - mx;
+                        int hi = min(csn[base + ka + 1], j1);
+                        while (j0 < hi) {
+                            const int mid = (j0 + hi) >> 1;
+                            const T xm = load_entry_x<T>(rs, (uint32_t)mid * (uint32_t)sizeof(Ent<T>));
+                            if (xm < xbeg) j0 = mid + 1;
+                            else hi = mid;
+                        }
                         if (MODE == MODE_A_ORTHO && selfrow) j0 = max(j0, pos + 1);
                         len = max(j1 - j0, 0);
                         xend = p.x + hw + mx;
