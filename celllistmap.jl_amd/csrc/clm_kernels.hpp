@@ -755,19 +755,22 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                     if (g[U - 1].x > xend) len = min(len, it + U);
                     return ballot64(it + U < len) != 0;
                 };
-                auto load = [&](Ent<T> *g, int it) {
-                    const uint32_t v = vb + (uint32_t)(it * sizeof(Ent<T>));
+                // one running byte offset per lane, advanced by a group per load (the
+                // entries of a group sit at constant offsets from it)
+                uint32_t vo = vb;
+                auto load = [&](Ent<T> *g) {
 #pragma unroll
-                    for (int u = 0; u < U; ++u) g[u] = load_entry<T>(rs, v + (uint32_t)(u * sizeof(Ent<T>)));
+                    for (int u = 0; u < U; ++u) g[u] = load_entry<T>(rs, vo + (uint32_t)(u * sizeof(Ent<T>)));
+                    vo += (uint32_t)(U * sizeof(Ent<T>));
                 };
                 // two buffers used in turn (no register copies): while one group is tested
                 // the next one is in flight
                 Ent<T> ga[U], gb[U];
-                load(ga, 0);
+                load(ga);
                 for (int it = 0; it < maxlen; it += 2 * U) {
-                    load(gb, it + U);
+                    load(gb);
                     if (!group(ga, it)) break;
-                    load(ga, it + 2 * U);
+                    load(ga);
                     if (!group(gb, it + U)) break;
                 }
             }
