@@ -185,7 +185,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--lcell", type=int, default=3)
+    ap.add_argument("--lcell", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ef-reps", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint comparisons")

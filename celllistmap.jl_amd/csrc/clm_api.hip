@@ -73,6 +73,10 @@ struct clm_ctx {
     int64_t n_entries = 0;
     DBuf xw, home_cell, cnt, creal, cstart, cur_real, cur_ghost, E, E_raw, rstart, real_pos, order, flags, cnt64, maxcnt;
     bool order_valid = false;  // order[] holds the particles in the cell order of the last build
+    // triclinic boxes: the cell-sorted entries split into the real ones and the ghost ones
+    // (written by k_cell_sort with E; split_valid when the last sort wrote them)
+    DBuf E_real, E_ghost;
+    bool split_valid = false;
     int bucket = 0;  // entries per cell bucket of the single-pass binning; 0 = not sized yet, -1 = off
     int64_t e_cap = 0;
     DBuf hc, ntask, hstart[3], tstart[3];
@@ -420,21 +424,41 @@ clm_status enqueue_count(clm_ctx *c) {
     return scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1);
 }
 
+// The triclinic sweep of one set walks the real and the ghost entries in separate arrays.
+bool wants_split(const clm_ctx *c) { return c->celltype == CLM_TRICLINIC && !c->root; }
+
+// Sizes the split arrays for the current entry capacity (before a sort is enqueued or
+// captured).
+template <typename T>
+clm_status ensure_split(clm_ctx *c) {
+    if (!wants_split(c)) return CLM_OK;
+    HIPCHK(c, ensure(c->E_real, sizeof(Ent<T>) * (std::max<int64_t>(c->n, 1) + SWEEP_PAD)));
+    HIPCHK(c, ensure(c->E_ghost, sizeof(Ent<T>) * (std::max<int64_t>(c->e_cap, 1) + SWEEP_PAD)));
+    return CLM_OK;
+}
+
 // Order every cell by x into E and list the real particles' positions (k_cell_sort); the
 // unsorted entries are in E_raw, compact (bucket 0) or in cell buckets.
 // threads: an upper bound of the entry count (the count itself when known on the host, the
 // entry capacity inside a captured graph).
 template <typename T>
 clm_status enqueue_sort(clm_ctx *c, int bucket, int64_t threads) {
+    c->split_valid = false;
     if (c->ncells <= 0 || threads <= 0) {
         c->order_valid = false;
         return CLM_OK;
     }
+    // the split arrays, when wanted and sized for this capacity (ensure_split)
+    const bool split = wants_split(c) && c->E_real.bytes >= sizeof(Ent<T>) * (size_t)std::max<int64_t>(c->n, 1) &&
+                       c->E_ghost.bytes >= sizeof(Ent<T>) * (size_t)std::max<int64_t>(c->e_cap, 1);
     LAUNCH(c, "k_cell_sort", hipLaunchKernelGGL((k_cell_sort<T>), dim3(blocks(threads, 256)), dim3(256), 0, c->stream,
                                                 c->ncells, (const int *)c->cstart.p, (const int *)c->rstart.p,
-                                                (const Ent<T> *)c->E_raw.p, bucket, (Ent<T> *)c->E.p,
-                                                (int *)c->real_pos.p, (int *)c->order.p));
+                                                (const Ent<T> *)c->E_raw.p, bucket, c->e_cap, (Ent<T> *)c->E.p,
+                                                (int *)c->real_pos.p, (int *)c->order.p,
+                                                split ? (Ent<T> *)c->E_real.p : nullptr,
+                                                split ? (Ent<T> *)c->E_ghost.p : nullptr));
     c->order_valid = true;
+    c->split_valid = split;
     return CLM_OK;
 }
 
@@ -533,6 +557,7 @@ clm_status bin_bucketed(clm_ctx *c, bool &done) {
         }
         HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
         c->e_cap = (int64_t)(c->E.bytes / sizeof(Ent<T>)) - SWEEP_PAD;
+        if ((s = ensure_split<T>(c))) return s;
         if ((s = enqueue_sort<T>(c, c->bucket, total))) return s;
         done = true;
         return CLM_OK;
@@ -579,6 +604,7 @@ clm_status update_cell_list_t(clm_ctx *c) {
         HIPCHK(c, ensure(c->E, sizeof(Ent<T>) * ((int64_t)total + SWEEP_PAD)));
         HIPCHK(c, ensure(c->E_raw, c->E.bytes));
         c->e_cap = (int64_t)(std::min(c->E.bytes, c->E_raw.bytes) / sizeof(Ent<T>)) - SWEEP_PAD;
+        if ((s = ensure_split<T>(c))) return s;
         if ((s = enqueue_scatter<T>(c))) return s;
         if (c->bucket == 0) c->bucket = bucket_size(c, mx, sizeof(Ent<T>));
     }
@@ -717,6 +743,14 @@ clm_status run_sweep_on(clm_ctx *c, clm_ctx *nb, const char *name, const Op &op,
     sa.en_bytes = (uint32_t)std::min<size_t>(nb->E.bytes, 0xFFFFFFF0u);
     if (c->sweep_variant == 1 && nb->E.bytes > 0xFFFFFFF0u)
         return fail(c, CLM_ERR_ARG, "cell list above 4 GiB: use sweep variant 0");
+    sa.reals = (const Ent<T> *)c->E_real.p;
+    sa.ghosts = (const Ent<T> *)c->E_ghost.p;
+    sa.reals_bytes = (uint32_t)std::min<size_t>(c->E_real.bytes, 0xFFFFFFF0u);
+    sa.ghosts_bytes = (uint32_t)std::min<size_t>(c->E_ghost.bytes, 0xFFFFFFF0u);
+    if (MODE == MODE_A_TRIC && c->sweep_variant == 1) {
+        if (!c->split_valid) return fail(c, CLM_ERR_STATE, "the split entry arrays of the triclinic sweep are not built");
+        if (c->E_ghost.bytes > 0xFFFFFFF0u) return fail(c, CLM_ERR_ARG, "cell list above 4 GiB: use sweep variant 0");
+    }
     if (c->sweep_variant == 1) {
         if (c->dim == 3)
             LAUNCH(c, name, hipLaunchKernelGGL((k_sweep_win<T, MODE, 3, Op>), dim3(grid), dim3(64), 0, c->stream, E, cs, En, csn, sa, op));
@@ -821,6 +855,7 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         op.total = tot;
         op.chunk_used = (int *)c->ch_used.p;
         op.max_chunks = nchunks;
+        op.ordered = mode_once(c) == MODE_A_TRIC;
         if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, grid))) return s;
         // compaction (a no-op on the device when the pairs exceed the capacity), then one
         // host synchronisation for the count and the overflow word
@@ -982,6 +1017,7 @@ clm_status graph_capture_t(clm_ctx *c, const clm_softcore *p, void *forces) {
         HIPCHK(c, ensure(c->E_raw, c->E.bytes));
         c->e_cap = (int64_t)(std::min(c->E.bytes, c->E_raw.bytes) / sizeof(Ent<T>)) - SWEEP_PAD;
     }
+    if ((s = ensure_split<T>(c))) return s;
     const int64_t nD = std::max<int64_t>(c->n * c->dim, 1);
     HIPCHK(c, ensure(c->outbuf, sizeof(T) * nD));
     HIPCHK(c, ensure(c->hc, sizeof(int) * (c->ncells + 1)));
@@ -1154,7 +1190,8 @@ clm_status clm_destroy(clm_ctx *c) {
                     &c->scan_tmp, &c->nl_i, &c->nl_j, &c->nl_d, &c->nl_count, &c->outbuf, &c->acc,
                     &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b, &c->sp_i, &c->sp_j, &c->sp_d,
                     &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart, &c->sys_desc,
-                    &c->sys_of, &c->rowoff, &c->E_raw, &c->rstart, &c->real_pos, &c->cnt64, &c->maxcnt, &c->order};
+                    &c->sys_of, &c->rowoff, &c->E_raw, &c->rstart, &c->real_pos, &c->cnt64, &c->maxcnt, &c->order,
+                    &c->E_real, &c->E_ghost};
     for (DBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

@@ -16,7 +16,11 @@
 //                 boxes: homes = all entries of cells holding a real particle, forward
 //                 stencil, later entries of the own cell, skip ghost-ghost pairs.
 //   MODE_A_TRIC   each pair once for triclinic boxes: homes = real particles, full
-//                 stencil, keep index_i < index_j.
+//                 stencil, keep index_i < index_j.  The windowed sweep takes the same
+//                 pairs in two parts: pairs of two real particles from the home earlier
+//                 in the cell order (a half stencil over the real entries alone), pairs
+//                 with a ghost image by the index rule (full stencil over the ghost
+//                 entries alone, rows without ghost images skipped).
 //   MODE_B        each pair from both sides (homes = real particles, full stencil,
 //                 index_i != index_j); operators accumulate only into the home
 //                 particle, so no atomics are needed for forces.
@@ -30,6 +34,8 @@
 //                 pair once per real home (the triclinic cross sweep, and the force
 //                 sweeps of either set for any box).
 #pragma once
+
+#include <type_traits>
 
 #include "clm_device.hpp"
 
@@ -337,16 +343,23 @@ __global__ void __launch_bounds__(256) k_bin_unpack(int ncells, const unsigned l
 // Ein holds the entries of cell c at cstart[c] (compact) or, when bucket > 0, at c * bucket.
 // The grid covers at least cstart[ncells] threads (the entry count, read on the device so
 // that a captured graph sorts whatever the count of its replay is, within the capacity).
+// Cells that do not end within cap entries are left out (their replay reports the
+// overflow raised by the scatter).
+// Rout / Gout (both null or both set): the same order split into the real entries alone
+// (Rout[rstart[c] + k], k-th real of cell c) and the ghost entries alone (Gout[cstart[c] -
+// rstart[c] + k], k-th ghost of cell c), the two arrays the triclinic sweep walks.
 template <typename T>
 __global__ void __launch_bounds__(256) k_cell_sort(int ncells, const int *__restrict__ cstart,
                                                    const int *__restrict__ rstart, const Ent<T> *__restrict__ Ein,
-                                                   int bucket, Ent<T> *__restrict__ Eout, int *__restrict__ real_pos,
-                                                   int *__restrict__ order) {
+                                                   int bucket, int64_t cap, Ent<T> *__restrict__ Eout,
+                                                   int *__restrict__ real_pos, int *__restrict__ order,
+                                                   Ent<T> *__restrict__ Rout, Ent<T> *__restrict__ Gout) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= cstart[ncells]) return;
     const int c = last_le(cstart, 0, ncells, k);  // the last cell starting at or before k: never empty
     const int s0 = cstart[c];
     const int n = cstart[c + 1] - s0;
+    if ((int64_t)s0 + n > cap) return;
     const Ent<T> *src = Ein + (bucket > 0 ? (size_t)c * bucket : (size_t)s0);
     const Ent<T> e = src[k - s0];
     const uint32_t we = ent_w(e);
@@ -359,10 +372,13 @@ __global__ void __launch_bounds__(256) k_cell_sort(int ncells, const int *__rest
         rrank += (less && !w_ghost(wf)) ? 1 : 0;
     }
     Eout[s0 + rank] = e;
+    const int r0 = rstart[c];
     if (!w_ghost(we)) {
-        const int r0 = rstart[c];
         real_pos[r0 + rrank] = s0 + rank;
         order[r0 + rrank] = (int)w_idx(we);
+        if (Rout) Rout[r0 + rrank] = e;
+    } else if (Gout) {
+        Gout[(s0 - r0) + (rank - rrank)] = e;
     }
 }
 
@@ -422,6 +438,11 @@ struct SweepArgs {
     int nrows;                 // rows over all systems
     int nsys;
     uint32_t en_bytes;         // bytes of the neighbour entry array (buffer-load range)
+    // MODE_A_TRIC (windowed sweep): the entries split by k_cell_sort into the real ones
+    // (cell starts hstart) and the ghost ones (cell starts cstart - hstart)
+    const Ent<T> *reals;
+    const Ent<T> *ghosts;
+    uint32_t reals_bytes, ghosts_bytes;
 };
 
 // Dynamic task fetch: blocks are dealt to XCDs round-robin (block b runs on XCD b % 8),
@@ -528,7 +549,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                             if (MODE == MODE_A_ORTHO) {
                                 ok = ok && (j >= lo) && (j < hi) && (preal || !w_ghost(wq));
                             } else if (MODE == MODE_A_TRIC) {
-                                ok = ok && (j >= lo) && (j < hi) && (wp < wq);
+                                ok = ok && (j >= lo) && (j < hi) && ((wp | 15u) < wq);  // index_i < index_j
                             } else if (MODE == MODE_X_ORTHO) {
                                 const bool same = (j >= slo) && (j < shi);
                                 ok = ok && (j >= lo) && (j < hi) && (preal || (!same && !w_ghost(wq)));
@@ -537,7 +558,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                             } else {
                                 ok = ok && valid && (wp != wq);
                             }
-                            op.candidate(sh, ws, ls, ok, p, q, d2);
+                            op.candidate(sh, ws, ls, ok, ballot64(ok), p, q, d2);
                         }
                     }
 #pragma unroll
@@ -645,7 +666,10 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
     constexpr int U = CLM_SWEEP_U;  // entries per lane per step (and as many prefetched)
     typename Op::Wave ws;
     op.wave_init(sh, ws);
-    const __amdgpu_buffer_rsrc_t rs = entry_rsrc(En, sa.en_bytes);
+    // MODE_A_TRIC walks two arrays (the real entries, the ghost entries), the others one
+    constexpr bool SPLIT = MODE == MODE_A_TRIC;
+    const __amdgpu_buffer_rsrc_t rs_main = SPLIT ? entry_rsrc(sa.reals, sa.reals_bytes) : entry_rsrc(En, sa.en_bytes);
+    const __amdgpu_buffer_rsrc_t rs_ghost = SPLIT ? entry_rsrc(sa.ghosts, sa.ghosts_bytes) : rs_main;
     const int ntasks = sa.tstart[sa.nrows];
     int home_xcd = blockIdx.x & 7, probe = 0;
     while (true) {
@@ -674,7 +698,12 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
         const int cx = c - rowc0;
         const int cy = lr % ncy, cz = lr / ncy;
         const uint32_t wp = ent_w(p);
+        const uint32_t wpx = wp | 15u;  // for a real home: index_p < index_q <=> wpx < wq
         const bool preal = !w_ghost(wp);
+        const uint64_t preal_mask = ballot64(preal);
+        // cells of the row reached by the wave (homes are consecutive along x; lane 0 is
+        // always a home)
+        const int cxlo = uniform_int(cx), cxhi = SPLIT ? wave_max_int(cx) : cxlo;
         const T mx = T(1e-3) * bx.cs[0], my = T(1e-3) * bx.cs[1], mz = T(1e-3) * bx.cs[2];
         const T invx = T(1) / bx.cs[0];
         typename Op::Lane ls;
@@ -685,93 +714,139 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
             for (int dy = -L; dy <= L; ++dy) {
                 const int base = sd.cell_off + ((cz + dz) * ncy + (cy + dy)) * ncx;
                 const bool selfrow = (dz == 0 && dy == 0);
-                int xlo = -L;
-                if (MODE == MODE_A_ORTHO) xlo = (dy > 0 || (dy == 0 && dz > 0)) ? 0 : 1;
-                const int k0 = cx + ((MODE == MODE_A_ORTHO && selfrow) ? 0 : xlo);
-                const int k1 = cx + L;
                 const T gy = slab_gap(bx, 1, cy + dy, p.y, my);
                 const T r2 = rc2 - gy * gy - gz * gz;
-                int j0 = 0, len = 0;
-                T xend = p.x;
-                if (valid && r2 >= T(0) && k0 <= k1) {
-                    const T hw = window_sqrt(r2) + mx;
-                    int ka = axis_cell(bx, 0, p.x - hw, invx);
-                    int kb = axis_cell(bx, 0, p.x + hw, invx);
+                // the lane's window: cells [ka, kb] of the row (before the stencil limits)
+                // and the x range [p.x - hw, p.x + hw]
+                const bool live = valid && r2 >= T(0);
+                T hw = T(0);
+                int ka = 0, kb = -1;
+                if (live) {
+                    hw = window_sqrt(r2) + mx;
+                    ka = axis_cell(bx, 0, p.x - hw, invx);
+                    kb = axis_cell(bx, 0, p.x + hw, invx);
                     if (kb == L - 1) kb = L;
                     if (ka == ncx - L) ka = ncx - L - 1;
-                    ka = max(ka, k0);
-                    kb = min(kb, k1);
-                    if (ka <= kb) {
-                        j0 = csn[base + ka];
-                        int j1 = csn[base + kb + 1];
+                }
+                // PASS 0: the entries of E (or, SPLIT, the real entries); PASS 1 (SPLIT
+                // only): the ghost entries
+                auto walk = [&](auto pass_tag) {
+                    constexpr int PASS = decltype(pass_tag)::value;
+                    const __amdgpu_buffer_rsrc_t rr = PASS == 1 ? rs_ghost : rs_main;
+                    auto start = [&](int k) -> int {
+                        if (!SPLIT) return csn[k];
+                        if (PASS == 0) return sa.hstart[k];
+                        return csn[k] - sa.hstart[k];
+                    };
+                    // SPLIT real entries: the half stencil that orders pairs by position
+                    // (rows after the own one, and the own row from the home on)
+                    const bool own_fwd = (MODE == MODE_A_ORTHO || (SPLIT && PASS == 0)) && selfrow;
+                    int xlo = -L;
+                    if (MODE == MODE_A_ORTHO) xlo = (dy > 0 || (dy == 0 && dz > 0)) ? 0 : 1;
+                    const int k0 = cx + (own_fwd ? 0 : xlo);
+                    const int k1 = cx + L;
+                    const int a = max(ka, k0), b = min(kb, k1);
+                    int j0 = 0, len = 0;
+                    T xend = p.x;
+                    if (live && a <= b) {
+                        j0 = start(base + a);
+                        const int j1 = start(base + b + 1);
                         // skip the entries of the first cell that lie before the window
                         // (the cell is ordered by x): bisection on their x
                         const T xbeg = p.x - hw - mx;
-                        int hi = min(csn[base + ka + 1], j1);
+                        int hi = min(start(base + a + 1), j1);
                         while (j0 < hi) {
                             const int mid = (j0 + hi) >> 1;
-                            const T xm = load_entry_x<T>(rs, (uint32_t)mid * (uint32_t)sizeof(Ent<T>));
+                            const T xm = load_entry_x<T>(rr, (uint32_t)mid * (uint32_t)sizeof(Ent<T>));
                             if (xm < xbeg) j0 = mid + 1;
                             else hi = mid;
                         }
                         if (MODE == MODE_A_ORTHO && selfrow) j0 = max(j0, pos + 1);
+                        if (SPLIT && PASS == 0 && selfrow) j0 = max(j0, h + 1);
                         len = max(j1 - j0, 0);
                         xend = p.x + hw + mx;
                     }
-                }
-                // MODE_X_ORTHO: entries of the same-index cell pair only with a real home
-                int slo = 0, shi = 0;
-                if (MODE == MODE_X_ORTHO && selfrow) { slo = csn[base + cx]; shi = csn[base + cx + 1]; }
-                const int maxlen = wave_max_int(len);
-                const uint32_t vb = (uint32_t)j0 * (uint32_t)sizeof(Ent<T>);
-                // One group of U entries: test them, then stop the lane at the window's end
-                // (the row is ordered by x: once an entry lies past it, the rest of the
-                // lane's range does too; entries in between are still tested).  Returns
-                // whether any lane has entries left.
-                auto group = [&](const Ent<T> *g, int it) -> bool {
+                    // the loads of this row have arrived (the bisection waited for them)
+                    op.row_sync(ws);
+                    // MODE_X_ORTHO: entries of the same-index cell pair only with a real home
+                    int slo = 0, shi = 0;
+                    if (MODE == MODE_X_ORTHO && selfrow) { slo = csn[base + cx]; shi = csn[base + cx + 1]; }
+                    const int maxlen = wave_max_int(len);
+                    const uint32_t vb = (uint32_t)j0 * (uint32_t)sizeof(Ent<T>);
+                    // One group of U entries: test them, then stop the lane at the window's end
+                    // (the row is ordered by x: once an entry lies past it, the rest of the
+                    // lane's range does too; entries in between are still tested).  Returns
+                    // whether any lane has entries left.
+                    auto group = [&](const Ent<T> *g, int it) -> bool {
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int t = it + u;
-                        const Ent<T> &q = g[u];
-                        const bool act = t < len;
-                        const uint32_t wq = ent_w(q);
-                        const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
-                        T d2 = dx * dx + dy2 * dy2;
-                        if (D == 3) d2 = d2 + dz2 * dz2;
-                        bool ok = act && d2 <= rc2;
-                        if (MODE == MODE_A_ORTHO) {
-                            ok = ok && (preal || !w_ghost(wq));
-                        } else if (MODE == MODE_A_TRIC) {
-                            ok = ok && (wp < wq);
-                        } else if (MODE == MODE_X_ORTHO) {
-                            const int j = j0 + t;
-                            const bool same = (j >= slo) && (j < shi);
-                            ok = ok && (preal || (!same && !w_ghost(wq)));
-                        } else if (MODE == MODE_B) {
-                            ok = ok && (wp != wq);
+                        for (int u = 0; u < U; ++u) {
+                            const int t = it + u;
+                            const Ent<T> &q = g[u];
+                            const bool act = t < len;
+                            const uint32_t wq = ent_w(q);
+                            const T dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
+                            T d2 = dx * dx + dy2 * dy2;
+                            if (D == 3) d2 = d2 + dz2 * dz2;
+                            // ok, and the wave's mask of it assembled from the ballots
+                            // of the single comparisons (no per-lane select)
+                            const bool in = d2 <= rc2;
+                            bool ok = act && in;
+                            uint64_t m = ballot64(act) & ballot64(in);
+                            if (MODE == MODE_A_ORTHO) {
+                                const bool real_q = !w_ghost(wq);
+                                ok = ok && (preal || real_q);
+                                m &= preal_mask | ballot64(real_q);
+                            } else if (MODE == MODE_A_TRIC) {
+                                if (PASS == 1) {
+                                    const bool lower = wpx < wq;
+                                    ok = ok && lower;
+                                    m &= ballot64(lower);
+                                }
+                            } else if (MODE == MODE_X_ORTHO) {
+                                const int j = j0 + t;
+                                const bool same = (j >= slo) && (j < shi);
+                                ok = ok && (preal || (!same && !w_ghost(wq)));
+                                m = ballot64(ok);
+                            } else if (MODE == MODE_B) {
+                                const bool other = wp != wq;
+                                ok = ok && other;
+                                m &= ballot64(other);
+                            }
+                            op.candidate(sh, ws, ls, ok, m, p, q, d2);
                         }
-                        op.candidate(sh, ws, ls, ok, p, q, d2);
-                    }
-                    if (g[U - 1].x > xend) len = min(len, it + U);
-                    return ballot64(it + U < len) != 0;
-                };
-                // one running byte offset per lane, advanced by a group per load (the
-                // entries of a group sit at constant offsets from it)
-                uint32_t vo = vb;
-                auto load = [&](Ent<T> *g) {
+                        if (g[U - 1].x > xend) len = min(len, it + U);
+                        return ballot64(it + U < len) != 0;
+                    };
+                    // one running byte offset per lane, advanced by a group per load (the
+                    // entries of a group sit at constant offsets from it)
+                    uint32_t vo = vb;
+                    auto load = [&](Ent<T> *g) {
 #pragma unroll
-                    for (int u = 0; u < U; ++u) g[u] = load_entry<T>(rs, vo + (uint32_t)(u * sizeof(Ent<T>)));
-                    vo += (uint32_t)(U * sizeof(Ent<T>));
-                };
-                // two buffers used in turn (no register copies): while one group is tested
-                // the next one is in flight
-                Ent<T> ga[U], gb[U];
-                load(ga);
-                for (int it = 0; it < maxlen; it += 2 * U) {
-                    load(gb);
-                    if (!group(ga, it)) break;
+                        for (int u = 0; u < U; ++u) g[u] = load_entry<T>(rr, vo + (uint32_t)(u * sizeof(Ent<T>)));
+                        vo += (uint32_t)(U * sizeof(Ent<T>));
+                    };
+                    // two buffers used in turn (no register copies): while one group is tested
+                    // the next one is in flight
+                    Ent<T> ga[U], gb[U];
                     load(ga);
-                    if (!group(gb, it + U)) break;
+                    for (int it = 0; it < maxlen; it += 2 * U) {
+                        load(gb);
+                        if (!group(ga, it)) break;
+                        load(ga);
+                        if (!group(gb, it + U)) break;
+                    }
+                };
+                if (SPLIT) {
+                    // each pair of real particles from the one earlier in the cell order;
+                    // each pair with a ghost image from the home of the lower index (the
+                    // reference's rule), over the full stencil -- skipped in rows without
+                    // ghost images within the wave's reach
+                    if (dz > 0 || (dz == 0 && dy >= 0)) walk(std::integral_constant<int, 0>{});
+                    const int g0 = base + max(cxlo - L, 0), g1 = base + min(cxhi + L + 1, ncx);
+                    const int ng = uniform_int((csn[g1] - sa.hstart[g1]) - (csn[g0] - sa.hstart[g0]));
+                    if (ng > 0) walk(std::integral_constant<int, 1>{});
+                } else {
+                    walk(std::integral_constant<int, 0>{});
                 }
             }
         }
@@ -820,6 +895,41 @@ struct NLOut {
             atomicOr(overflow, 1u);
         }
     }
+    // The wave's staged pairs li/lj/ld2[0, m) into the consecutive slots [s, s + m) (s
+    // and m wave-uniform): the run lies in the caller's arrays or in the spill area as a
+    // whole, so the destination is chosen once per run; only a run across the end of
+    // either takes the per-slot path.  ordered: each pair as (lower index, higher index).
+    __device__ void put_run(long long s, int m, const int *li, const int *lj, const T *ld2, bool ordered) const {
+        int32_t *pi, *pj;
+        T *pd;
+        if (s + m <= cap) {
+            pi = oi + s; pj = oj + s; pd = od + s;
+        } else if (s >= cap && s - cap + m <= scap) {
+            pi = si + (s - cap); pj = sj + (s - cap); pd = sd + (s - cap);
+        } else {
+            for (int t = threadIdx.x; t < m; t += 64) {
+                int a = li[t], b = lj[t];
+                if (ordered) {
+                    const int lo = min(a, b);
+                    b = max(a, b);
+                    a = lo;
+                }
+                put(s + t, a, b, dev_sqrt_rn(ld2[t]));
+            }
+            return;
+        }
+        for (int t = threadIdx.x; t < m; t += 64) {
+            int a = li[t], b = lj[t];
+            if (ordered) {
+                const int lo = min(a, b);
+                b = max(a, b);
+                a = lo;
+            }
+            pi[t] = a;
+            pj[t] = b;
+            pd[t] = dev_sqrt_rn(ld2[t]);
+        }
+    }
     __device__ void get(long long s, int &i, int &j, T &d) const {
         const bool u = s < cap;
         const long long q = u ? s : s - cap;
@@ -858,12 +968,15 @@ struct OpNeighborList {
         int nxt_k;                   // counter of the pending request
         int req;                     // requests made so far
         bool have_nxt;
+        bool nxt_ready;              // the pending request's chunk is known (nxt_val)
+        long long nxt_val;
     };
     NLOut<T> out;
     unsigned long long *chunk_counter;  // NL_COUNTERS counters, NL_COUNTER_STRIDE words apart
     unsigned long long *total;
     int *chunk_used;
     long long max_chunks;
+    int ordered;  // write each pair as (lower index, higher index)
 
     __device__ void request(Wave &w) const {
         const int k = (int)((blockIdx.x + (unsigned)w.req) % NL_COUNTERS);
@@ -871,9 +984,20 @@ struct OpNeighborList {
         w.nxt_k = k;
         w.req += 1;
         w.have_nxt = true;
+        w.nxt_ready = false;
     }
     __device__ long long granted(const Wave &w) const {
+        if (w.nxt_ready) return w.nxt_val;
         return (long long)uniform_u64(w.nxt_raw) * NL_COUNTERS + w.nxt_k;
+    }
+    // Called by the sweep once the loads that start a row have arrived: vector memory
+    // operations complete in issue order, so a pending request issued before them is
+    // complete as well, and reading its result here costs no wait of its own.
+    __device__ void row_sync(Wave &w) const {
+        if (w.have_nxt && !w.nxt_ready) {
+            w.nxt_val = (long long)uniform_u64(w.nxt_raw) * NL_COUNTERS + w.nxt_k;
+            w.nxt_ready = true;
+        }
     }
     __device__ void wave_init(Shared &, Wave &w) const {
         w.staged = 0;
@@ -884,6 +1008,8 @@ struct OpNeighborList {
         w.nxt_k = 0;
         w.req = 0;
         w.have_nxt = false;  // the first chunk is requested at the first flush
+        w.nxt_ready = false;
+        w.nxt_val = 0;
     }
     __device__ void lane_init(Wave &w, Lane &l, const Ent<T> &, bool) const { l.n = w.staged; }
     __device__ void retire(const Wave &w, int used) const {
@@ -902,8 +1028,7 @@ struct OpNeighborList {
             }
             const int m = min(n - k, CH - w.cur_used);
             const long long base = w.cur * CH + w.cur_used;
-            for (int t = threadIdx.x; t < m; t += 64)
-                out.put(base + t, sh.i[k + t], sh.j[k + t], dev_sqrt_rn(sh.d2[k + t]));
+            out.put_run(base, m, sh.i + k, sh.j + k, sh.d2 + k, ordered != 0);
             w.cur_used += m;
             k += m;
         }
@@ -911,8 +1036,8 @@ struct OpNeighborList {
         if (!w.have_nxt && w.cur_used > CH - PREFETCH) request(w);
         __syncthreads();
     }
-    __device__ void candidate(Shared &sh, Wave &w, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
-        const uint64_t m = ballot64(ok);
+    __device__ void candidate(Shared &sh, Wave &w, Lane &l, bool ok, uint64_t m, const Ent<T> &p, const Ent<T> &q,
+                              T d2) const {
         if (ok) {
             const int k = l.n + rank_in(m);
             sh.i[k] = (int)w_idx(ent_w(p));
@@ -1035,7 +1160,8 @@ struct OpSoftCore {
         l.aeps = l.asig = T(0);
         l.ti = prm.ntypes ? (int)w_type(ent_w(p)) : 0;
     }
-    __device__ void candidate(Shared &sh, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
+    __device__ void candidate(Shared &sh, Wave &, Lane &l, bool ok, uint64_t, const Ent<T> &p, const Ent<T> &q,
+                              T d2) const {
         if (!ok) return;
         int t = 0;
         if (prm.ntypes) t = l.ti * 8 + (int)w_type(ent_w(q));
@@ -1057,6 +1183,7 @@ struct OpSoftCore {
             l.asig += e * w / sh.sig[t];
         }
     }
+    __device__ void row_sync(Wave &) const {}
     __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &p, bool valid, const SysDesc<T> &sd,
                              int si) const {
         double ev = valid ? escale * (double)l.e : 0.0;
@@ -1132,7 +1259,8 @@ struct OpHistogram {
             l.vx = o[0]; l.vy = o[1]; l.vz = o[2];
         }
     }
-    __device__ void candidate(Shared &sh, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
+    __device__ void candidate(Shared &sh, Wave &, Lane &l, bool ok, uint64_t, const Ent<T> &p, const Ent<T> &q,
+                              T d2) const {
         if (!ok) return;
         const T d = dev_sqrt_rn(d2);
         T bf = dev_floor(dev_div_rn(d, width));
@@ -1147,6 +1275,7 @@ struct OpHistogram {
             if (d > T(0)) atomicAdd(&sh.vs[b], (double)(t / d));
         }
     }
+    __device__ void row_sync(Wave &) const {}
     __device__ void lane_end(Shared &, Wave &, Lane &, const Ent<T> &, bool, const SysDesc<T> &, int) const {}
     __device__ void wave_end(Shared &sh, Wave &) const {
         __syncthreads();
@@ -1185,12 +1314,14 @@ struct OpMinDist {
         l.d2 = T(INFINITY);
         l.a = l.b = 0x7fffffff;
     }
-    __device__ void candidate(Shared &, Wave &, Lane &l, bool ok, const Ent<T> &p, const Ent<T> &q, T d2) const {
+    __device__ void candidate(Shared &, Wave &, Lane &l, bool ok, uint64_t, const Ent<T> &p, const Ent<T> &q,
+                              T d2) const {
         if (!ok) return;
         int i = (int)w_idx(ent_w(p)), j = (int)w_idx(ent_w(q));
         int a = (cross || i < j) ? i : j, b = (cross || i < j) ? j : i;
         if (better(d2, a, b, l.d2, l.a, l.b)) { l.d2 = d2; l.a = a; l.b = b; }
     }
+    __device__ void row_sync(Wave &) const {}
     __device__ void lane_end(Shared &sh, Wave &, Lane &l, const Ent<T> &, bool, const SysDesc<T> &, int) const {
         T d2 = l.d2;
         int a = l.a, b = l.b;
