@@ -890,7 +890,9 @@ struct NLOut {
         int32_t *pj = u ? oj : sj;
         T *pd = u ? od : sd;
         if (u || q < scap) {
-            pi[q] = i; pj[q] = j; pd[q] = d;
+            __builtin_nontemporal_store(i, pi + q);
+            __builtin_nontemporal_store(j, pj + q);
+            __builtin_nontemporal_store(d, pd + q);
         } else {
             atomicOr(overflow, 1u);
         }
@@ -899,6 +901,8 @@ struct NLOut {
     // and m wave-uniform): the run lies in the caller's arrays or in the spill area as a
     // whole, so the destination is chosen once per run; only a run across the end of
     // either takes the per-slot path.  ordered: each pair as (lower index, higher index).
+    // The list is far larger than the caches and not read again by the sweep, so it is
+    // written with streaming (non-temporal) stores.
     __device__ void put_run(long long s, int m, const int *li, const int *lj, const T *ld2, bool ordered) const {
         int32_t *pi, *pj;
         T *pd;
@@ -925,9 +929,9 @@ struct NLOut {
                 b = max(a, b);
                 a = lo;
             }
-            pi[t] = a;
-            pj[t] = b;
-            pd[t] = dev_sqrt_rn(ld2[t]);
+            __builtin_nontemporal_store(a, pi + t);
+            __builtin_nontemporal_store(b, pj + t);
+            __builtin_nontemporal_store(dev_sqrt_rn(ld2[t]), pd + t);
         }
     }
     __device__ void get(long long s, int &i, int &j, T &d) const {
