@@ -1019,7 +1019,13 @@ struct OpNeighborList {
     __device__ void retire(const Wave &w, int used) const {
         if (w.cur >= 0 && w.cur < max_chunks && threadIdx.x == 0) chunk_used[w.cur] = used;
     }
-    __device__ void flush(Shared &sh, Wave &w, int n) const {
+    // Writes the first n staged pairs, rounded down to a multiple of 32 unless all: chunks
+    // start on 512-slot boundaries, so every run then starts and ends on whole cache lines
+    // of each output array.  The pairs left over (fewer than 32) move to the front of the
+    // staging arrays; returns their count.
+    __device__ int flush(Shared &sh, Wave &w, int n, bool all) const {
+        const int r = all ? 0 : (n & 31);
+        n -= r;
         __syncthreads();
         int k = 0;
         while (k < n) {
@@ -1039,6 +1045,16 @@ struct OpNeighborList {
         w.written += (unsigned long long)n;
         if (!w.have_nxt && w.cur_used > CH - PREFETCH) request(w);
         __syncthreads();
+        if (r > 0) {  // n >= 32 > r here: the source and destination do not overlap
+            const int t = threadIdx.x;
+            int a = 0, b = 0;
+            T d = T(0);
+            if (t < r) { a = sh.i[n + t]; b = sh.j[n + t]; d = sh.d2[n + t]; }
+            __syncthreads();
+            if (t < r) { sh.i[t] = a; sh.j[t] = b; sh.d2[t] = d; }
+            __syncthreads();
+        }
+        return r;
     }
     __device__ void candidate(Shared &sh, Wave &w, Lane &l, bool ok, uint64_t m, const Ent<T> &p, const Ent<T> &q,
                               T d2) const {
@@ -1050,15 +1066,14 @@ struct OpNeighborList {
         }
         l.n += __popcll(m);
         if (l.n > S - 64) {
-            flush(sh, w, l.n);
-            l.n = 0;
+            l.n = flush(sh, w, l.n, false);
         }
     }
     __device__ void lane_end(Shared &, Wave &w, Lane &l, const Ent<T> &, bool, const SysDesc<T> &, int) const {
         w.staged = l.n;
     }
     __device__ void wave_end(Shared &sh, Wave &w) const {
-        if (w.staged > 0) flush(sh, w, w.staged);
+        if (w.staged > 0) flush(sh, w, w.staged, true);
         if (w.cur >= 0) retire(w, w.cur_used);
         if (w.have_nxt) {
             const long long nx = granted(w);
