@@ -246,8 +246,9 @@ def main():
     log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step, {P:.4g} pairs")
 
     kernels = {}
-    for name in ("k_sweep_neighborlist", "k_bin_bucket", "k_bin_unpack", "k_bin_count", "k_bin_scatter", "k_count_max",
-                 "k_cell_sort", "scan", "k_fill_int", "k_home_counts", "k_row_tasks", "k_nl_holes", "k_nl_fill"):
+    for name in ("k_sweep_neighborlist", "k_bin_bucket", "k_scan_cells", "k_bin_count", "k_bin_scatter", "k_count_max",
+                 "k_cell_sort", "scan", "k_fill_int", "k_fill_multi", "k_home_counts", "k_scan_rows", "k_scan_holes",
+                 "k_nl_fill"):
         tot, cnt = kernel_stats(ctx, name)
         if cnt:
             kernels[name] = {"avg_ms": tot / cnt, "launches": cnt, "ms_per_step": tot / args.steps}

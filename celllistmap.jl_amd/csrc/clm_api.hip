@@ -2,9 +2,10 @@
 //
 // One context owns a HIP stream, the particle data and grow-only device buffers.
 // The pipeline per cell-list update is
-//   k_bin_count -> exclusive scan of cell counts -> k_bin_scatter
-// and per operator call
-//   (k_home_counts -> scan -> k_row_tasks -> scan, cached per sweep mode) -> k_sweep<Op>.
+//   k_bin_bucket -> k_scan_pair (cell counts) -> k_cell_sort
+// (or, on the first build of a layout and in captured graphs, the two passes k_bin_count ->
+// scans -> k_bin_scatter -> k_cell_sort), and per operator call
+//   ([k_home_counts -> scan] -> k_scan_pair (row tasks), cached per sweep mode) -> sweep.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -82,6 +83,7 @@ struct clm_ctx {
     DBuf hc, ntask, hstart[3], tstart[3];
     bool mode_valid[3] = {false, false, false};
     DBuf counters, scan_tmp;
+    DBuf scan_st[3];  // state of the single-pass scans: cells, rows, neighbour-list chunks
 
     // operator buffers
     DBuf nl_i, nl_j, nl_d, nl_count, outbuf, acc, hist_c, hist_v, md_d2, md_a, md_b;
@@ -400,6 +402,24 @@ clm_status ensure_bin_buffers(clm_ctx *c) {
 
 const int *sys_of_ptr(clm_ctx *c) { return c->nsys > 1 ? (const int *)c->sys_of.p : nullptr; }
 
+enum { SCAN_CELLS = 0, SCAN_ROWS = 1, SCAN_CHUNKS = 2 };
+
+// State of one single-pass scan site for n elements: the ticket and done counters, then
+// one status word per tile.  Zeroed when allocated (never inside a graph capture: the
+// capture sizes it first); every launch leaves it zeroed again.
+clm_status scan_state(clm_ctx *c, int site, long long n, ScanState &st) {
+    const size_t bytes = sizeof(unsigned long long) * (size_t)(scan_tiles(n) + 1);
+    DBuf &b = c->scan_st[site];
+    if (!b.p || b.bytes < bytes) {
+        HIPCHK(c, ensure(b, bytes));
+        HIPCHK(c, hipMemsetAsync(b.p, 0, b.bytes, c->stream));
+    }
+    st.ticket = (unsigned int *)b.p;
+    st.done = st.ticket + 1;
+    st.status = (unsigned long long *)b.p + 1;
+    return CLM_OK;
+}
+
 // k_bin_count + scan of the cell counts (no host synchronisation).
 template <typename T>
 clm_status enqueue_count(clm_ctx *c) {
@@ -491,6 +511,7 @@ clm_status enqueue_scatter(clm_ctx *c) {
 }
 
 clm_status check_flags(clm_ctx *c, uint32_t fl) {
+    if (fl & FLAG_SCAN) return fail(c, CLM_ERR_HIP, "a single-pass scan did not complete (internal error)");
     if (fl & FLAG_NAN) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN or Inf)");
     if (fl & FLAG_RANGE)
         return fail(c, CLM_ERR_RANGE, "a particle maps outside the computing grid (the reference raises a BoundsError)");
@@ -516,7 +537,6 @@ template <typename T>
 clm_status bin_bucketed(clm_ctx *c, bool &done) {
     done = false;
     const int64_t n = c->n;
-    const int nc1 = c->ncells + 1;
     HIPCHK(c, ensure(c->cnt64, sizeof(unsigned long long) * std::max(c->ncells, 1)));
     HIPCHK(c, ensure(c->maxcnt, sizeof(int) * 2));
     const SysDesc<T> *sd = (const SysDesc<T> *)c->sys_desc.p;
@@ -540,12 +560,14 @@ clm_status bin_bucketed(clm_ctx *c, bool &done) {
                                                              (unsigned long long *)c->cnt64.p, (Ent<T> *)c->E_raw.p,
                                                              c->bucket, (uint32_t *)c->flags.p));
         }
-        LAUNCH(c, "k_bin_unpack", hipLaunchKernelGGL(k_bin_unpack, dim3(std::min<unsigned>(blocks(nc1, 256), 128)), dim3(256), 0,
-                                                     c->stream, c->ncells,
-                                                     (const unsigned long long *)c->cnt64.p, (int *)c->cnt.p,
-                                                     (int *)c->creal.p, (int *)c->flags.p + 1, (int *)c->flags.p + 2));
-        if ((s = scan_int(c, (const int *)c->creal.p, (int *)c->rstart.p, nc1))) return s;
-        if ((s = scan_int(c, (const int *)c->cnt.p, (int *)c->cstart.p, nc1))) return s;
+        // per-cell counts, their scans, the fullest cell and the total in one launch
+        ScanState st;
+        if ((s = scan_state(c, SCAN_CELLS, c->ncells, st))) return s;
+        ProdCells pc{(const unsigned long long *)c->cnt64.p, (int *)c->cnt.p, (int *)c->creal.p};
+        LAUNCH(c, "k_scan_cells", hipLaunchKernelGGL((k_scan_pair<ProdCells>), dim3((unsigned)scan_tiles(c->ncells)),
+                                                     dim3(SCAN_THREADS), 0, c->stream, (long long)c->ncells, pc,
+                                                     (const unsigned long long *)nullptr, (int *)c->cstart.p, (int *)c->rstart.p, (int *)c->flags.p + 1,
+                                                     (int *)c->flags.p + 2, st, (unsigned int *)c->flags.p, FLAG_SCAN));
         HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         if ((s = sync(c))) return s;
         if ((s = check_flags(c, c->h_flags[0]))) return s;
@@ -679,10 +701,15 @@ clm_status enqueue_mode(clm_ctx *c, int mode) {
                                                       (int *)c->hc.p));
         if ((s = scan_int(c, (const int *)c->hc.p, (int *)c->hstart[mode].p, nc1))) return s;
     }
-    LAUNCH(c, "k_row_tasks", hipLaunchKernelGGL((k_row_tasks<T>), dim3(blocks(c->nrows + 1, 256)), dim3(256), 0, c->stream,
-                                                c->nrows, c->nsys, (const SysDesc<T> *)c->sys_desc.p,
-                                                (const int *)c->rowoff.p, home_start(c, mode), (int *)c->ntask.p));
-    return scan_int(c, (const int *)c->ntask.p, (int *)c->tstart[mode].p, c->nrows + 1);
+    // wave tasks per row and their scan in one launch
+    ScanState st;
+    if ((s = scan_state(c, SCAN_ROWS, c->nrows, st))) return s;
+    ProdRows<T> pr{(const SysDesc<T> *)c->sys_desc.p, (const int *)c->rowoff.p, home_start(c, mode), c->nsys};
+    LAUNCH(c, "k_scan_rows", hipLaunchKernelGGL((k_scan_pair<ProdRows<T>>), dim3((unsigned)scan_tiles(c->nrows)),
+                                                dim3(SCAN_THREADS), 0, c->stream, (long long)c->nrows, pr,
+                                                (const unsigned long long *)nullptr, (int *)c->tstart[mode].p, (int *)nullptr, (int *)nullptr, (int *)nullptr, st,
+                                                (unsigned int *)c->flags.p, FLAG_SCAN));
+    return CLM_OK;
 }
 
 clm_status prepare_mode(clm_ctx *c, int mode) {
@@ -859,11 +886,13 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, grid))) return s;
         // compaction (a no-op on the device when the pairs exceed the capacity), then one
         // host synchronisation for the count and the overflow word
-        LAUNCH(c, "k_nl_holes", hipLaunchKernelGGL(k_nl_holes, dim3(blocks(nchunks + 1, 256)), dim3(256), 0, c->stream,
-                                                   (long long)nchunks, (const int *)c->ch_used.p, tot, (long long)cap,
-                                                   (int *)c->ch_holes.p, (int *)c->ch_movers.p));
-        if ((s = scan_int(c, (const int *)c->ch_holes.p, (int *)c->ch_hstart.p, (int)nchunks + 1))) return s;
-        if ((s = scan_int(c, (const int *)c->ch_movers.p, (int *)c->ch_mstart.p, (int)nchunks + 1))) return s;
+        ScanState st;
+        if ((s = scan_state(c, SCAN_CHUNKS, nchunks, st))) return s;
+        ProdHoles ph{(const int *)c->ch_used.p, (long long)cap};
+        LAUNCH(c, "k_scan_holes", hipLaunchKernelGGL((k_scan_pair<ProdHoles>), dim3((unsigned)scan_tiles(nchunks)),
+                                                     dim3(SCAN_THREADS), 0, c->stream, (long long)nchunks, ph, tot,
+                                                     (int *)c->ch_hstart.p, (int *)c->ch_mstart.p, (int *)nullptr,
+                                                     (int *)nullptr, st, ovf, 2u));
         LAUNCH(c, "k_nl_fill", hipLaunchKernelGGL((k_nl_fill<T>), dim3(std::max(1, std::min(2048, c->num_cus * 8))), dim3(256),
                                                   0, c->stream, (long long)nchunks, (const int *)c->ch_used.p,
                                                   (const int *)c->ch_hstart.p, (const int *)c->ch_mstart.p, tot, out));
@@ -872,6 +901,7 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         if ((s = sync(c))) return s;
         std::memcpy(&cnt, c->h_flags, sizeof(cnt));
         const unsigned int h_ovf = c->h_flags[2];
+        if (h_ovf & 2u) return fail(c, CLM_ERR_HIP, "a single-pass scan did not complete (internal error)");
         if (!h_ovf || (int64_t)cnt > cap) break;
         if (attempt >= 3) return fail(c, CLM_ERR_HIP, "neighbor list spill area overflow");
         scap *= 2;
@@ -1026,6 +1056,10 @@ clm_status graph_capture_t(clm_ctx *c, const clm_softcore *p, void *forces) {
     HIPCHK(c, ensure(c->tstart[MODE_B], sizeof(int) * (c->nrows + 1)));
     HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
     HIPCHK(c, ensure(c->acc, sizeof(double) * (c->nsys + 16)));
+    {  // the row-task scan state (zeroed here, outside the capture)
+        ScanState st;
+        if ((s = scan_state(c, SCAN_ROWS, c->nrows, st))) return s;
+    }
     {  // size the scan scratch for the largest scan before capturing
         size_t need = 0;
         hipcub::DeviceScan::ExclusiveSum(nullptr, need, (const int *)nullptr, (int *)nullptr, c->ncells + 1, c->stream);
@@ -1191,7 +1225,7 @@ clm_status clm_destroy(clm_ctx *c) {
                     &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b, &c->sp_i, &c->sp_j, &c->sp_d,
                     &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart, &c->sys_desc,
                     &c->sys_of, &c->rowoff, &c->E_raw, &c->rstart, &c->real_pos, &c->cnt64, &c->maxcnt, &c->order,
-                    &c->E_real, &c->E_ghost};
+                    &c->E_real, &c->E_ghost, &c->scan_st[0], &c->scan_st[1], &c->scan_st[2]};
     for (DBuf *b : bufs)
         if (b->p) hipFree(b->p);
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

@@ -50,7 +50,7 @@ struct DevBox {
 };
 
 // Flags raised by the binning kernels (device word, read back by the host).
-enum : uint32_t { FLAG_NAN = 1u, FLAG_RANGE = 2u, FLAG_OVERFLOW = 4u };
+enum : uint32_t { FLAG_NAN = 1u, FLAG_RANGE = 2u, FLAG_OVERFLOW = 4u, FLAG_SCAN = 8u };
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
 

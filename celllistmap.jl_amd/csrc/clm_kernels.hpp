@@ -1,11 +1,16 @@
 // clm_kernels.hpp -- HIP kernels of the cell-list engine for gfx950.
 //
-//   k_bin_count   wrap + rotate each particle, count it and its ghost images per cell
-//                 (add_particles! / replicate_particle!, CellLists.jl:940-1057, Box.jl:556-566)
-//   k_bin_scatter write the cell-sorted entries (reals first in each cell)
-//   k_home_counts / k_row_tasks   per-cell home counts and per-row wave tasks
-//   k_sweep<Op>   the cell-pair sweep (self.jl:106-184, vicinal_cells.jl:4-75) with a
-//                 fixed device operator Op in place of the user closure
+//   k_bin_bucket  wrap + rotate each particle, place it and its ghost images in the
+//                 buckets of their cells (add_particles! / replicate_particle!,
+//                 CellLists.jl:940-1057, Box.jl:556-566); k_bin_count / k_bin_scatter
+//                 are the two-pass form
+//   k_scan_pair   single-pass scans: cell counts, per-row wave tasks, output chunks
+//   k_cell_sort   order every cell by x (and split real / ghost entries)
+//   k_home_counts per-cell home counts of the modes whose homes are all entries
+//   k_sweep_win<Op> / k_sweep<Op>   the cell-pair sweep (self.jl:106-184,
+//                 vicinal_cells.jl:4-75) with a fixed device operator Op in place of the
+//                 user closure
+//   k_nl_fill     compaction of the chunked neighbour-list output
 //
 // Sweep mapping: one wave (64 lanes) takes up to 64 consecutive home entries of one
 // x-row of cells.  For each of the (2L+1)^2 (3D) stencil rows the union of the lanes'
@@ -300,37 +305,6 @@ __global__ void __launch_bounds__(256) k_count_max(int ncells, const int *__rest
     block_max_to(mx, maxcnt);
 }
 
-// Splits the bucket counter words into entry and real counts (with the trailing zero the
-// scans need) and records the fullest cell and the entry total.  Grid-stride, 256 threads
-// per block.
-__global__ void __launch_bounds__(256) k_bin_unpack(int ncells, const unsigned long long *__restrict__ cnt64,
-                                                    int *__restrict__ cnt, int *__restrict__ creal,
-                                                    int *__restrict__ maxcnt, int *__restrict__ total) {
-    int mx = 0, sum = 0;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncells; c += gridDim.x * blockDim.x) {
-        if (c == ncells) {
-            cnt[c] = 0;
-            creal[c] = 0;
-            continue;
-        }
-        const unsigned long long v = cnt64[c];
-        const int ne = (int)(uint32_t)v;
-        cnt[c] = ne;
-        creal[c] = (int)(uint32_t)(v >> 32);
-        mx = max(mx, ne);
-        sum += ne;
-    }
-    __shared__ int ws[4];
-    sum = wave_sum(sum);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = sum;
-    block_max_to(mx, maxcnt);  // synchronises the block
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += ws[w];
-        if (t) atomicAdd(total, t);
-    }
-}
-
 // Orders the entries of every cell by (x, w) -- x in the aligned frame, w breaking ties
 // (unique per entry) -- and records where each real particle landed: real_pos[rstart[c]
 // + k] = position of the k-th real of cell c, order[rstart[c] + k] = its particle index
@@ -393,19 +367,6 @@ __global__ void k_home_counts(int ncells, int all_entries, const int *__restrict
 }
 
 
-template <typename T>
-__global__ void k_row_tasks(int nrows, int nsys, const SysDesc<T> *__restrict__ sys, const int *__restrict__ rowoff,
-                            const int *__restrict__ hstart, int *__restrict__ ntask) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r > nrows) return;
-    if (r == nrows) { ntask[r] = 0; return; }
-    const int s = last_le(rowoff, 0, nsys + 1, r);
-    const int ncx = sys[s].box.nc[0];
-    const int c0 = sys[s].cell_off + (r - rowoff[s]) * ncx;
-    int h = hstart[c0 + ncx] - hstart[c0];
-    ntask[r] = (h + 63) >> 6;
-}
-
 // Zeroes up to 8 int arrays in one launch (grid-stride).
 struct FillList {
     int *p[8];
@@ -422,6 +383,186 @@ __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+
+// ------------------------------------------------------------------------------
+// single-pass scans
+// ------------------------------------------------------------------------------
+//
+// Exclusive scans of two counts per element, produced on the fly by Prod::get(i, a, b),
+// in one launch: each workgroup takes a tile of SCAN_TILE elements in the order it
+// started (a ticket counter), publishes the tile's sums, then walks back over the
+// preceding tiles' published words until it meets a tile that published its inclusive
+// prefix ("decoupled look-back").  A tile only waits on tiles that started before it and
+// publish without waiting, so the walk ends; it is also bounded, and a walk that hits the
+// bound raises *err (the result is then invalid and the caller reports it).  The last
+// workgroup to finish clears the status words and counters, so the state is ready for the
+// next launch, also inside a captured graph.  out_a[n] / out_b[n] receive the totals.
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+constexpr unsigned long long SCAN_AGG = 1ull << 62, SCAN_PRE = 2ull << 62;
+constexpr unsigned int SCAN_SPIN_LIMIT = 1u << 24;
+
+struct ScanState {
+    unsigned long long *status;  // one word per tile: flag (2 bits) | a (31 bits) | b (31 bits)
+    unsigned int *ticket;
+    unsigned int *done;
+};
+
+inline long long scan_tiles(long long n) { return n / SCAN_TILE + 1; }
+
+__device__ inline unsigned long long scan_pack(int a, int b) {
+    return ((unsigned long long)(unsigned)a << 31) | (unsigned long long)(unsigned)b;
+}
+
+// uval (may be null): one device value every element needs, read once per workgroup and
+// passed to Prod::get (one address read by every thread would queue on one cache channel).
+template <typename Prod>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_pair(long long n, Prod prod, const unsigned long long *uval,
+                                                            int *__restrict__ out_a, int *__restrict__ out_b, int *maxa,
+                                                            int *tot_a, ScanState st, unsigned int *err,
+                                                            unsigned int err_bit) {
+    __shared__ int s_tile, s_pa, s_pb, s_last;
+    __shared__ unsigned long long s_u;
+    __shared__ int s_wa[SCAN_THREADS / 64], s_wb[SCAN_THREADS / 64], s_wm[SCAN_THREADS / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (t == 0) {
+        s_tile = (int)atomicAdd(st.ticket, 1u);
+        s_u = uval ? *uval : 0ull;
+    }
+    __syncthreads();
+    const int tile = s_tile;
+    const unsigned long long u = s_u;
+    const long long base = (long long)tile * SCAN_TILE + (long long)t * SCAN_ITEMS;
+    int va[SCAN_ITEMS], vb[SCAN_ITEMS];
+    int sa = 0, sb = 0, mx = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        int a = 0, b = 0;
+        if (base + k < n) prod.get(base + k, u, a, b);
+        va[k] = a;
+        vb[k] = b;
+        sa += a;
+        sb += b;
+        mx = max(mx, a);
+    }
+    // inclusive scan of the thread sums within the wave, then across the waves
+    int ia = sa, ib = sb;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int xa = __shfl_up(ia, off, 64), xb = __shfl_up(ib, off, 64);
+        if (lane >= off) { ia += xa; ib += xb; }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+    if (lane == 63) { s_wa[wv] = ia; s_wb[wv] = ib; s_wm[wv] = mx; }
+    __syncthreads();
+    int wa = 0, wb = 0, A = 0, B = 0, M = 0;
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+        if (w < wv) { wa += s_wa[w]; wb += s_wb[w]; }
+        A += s_wa[w];
+        B += s_wb[w];
+        M = max(M, s_wm[w]);
+    }
+    if (wv == 0) {
+        // publish, then look back with the whole wave: lane l reads the word of tile
+        // (window start - l); the nearest tile with a published prefix ends the walk
+        if (lane == 0)
+            __hip_atomic_store(&st.status[tile], (tile == 0 ? SCAN_PRE : SCAN_AGG) | scan_pack(A, B), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        int pa = 0, pb = 0;
+        int kb = tile - 1;
+        unsigned int spins = 0;
+        while (kb >= 0) {
+            const int k = kb - lane;
+            const unsigned long long v =
+                k >= 0 ? __hip_atomic_load(&st.status[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SCAN_PRE;
+            const unsigned long long f = v >> 62;
+            const uint64_t pre = ballot64(f == 2);
+            const uint64_t pending = ballot64(f == 0);
+            const int lp = pre ? __builtin_ctzll(pre) : 64;
+            const uint64_t upto = lp >= 63 ? ~0ull : ((2ull << lp) - 1);
+            if (pending & upto) {
+                if (++spins > SCAN_SPIN_LIMIT) {
+                    if (lane == 0) atomicOr(err, err_bit);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            int a = lane <= lp ? (int)((v >> 31) & 0x7fffffffu) : 0;
+            int b = lane <= lp ? (int)(v & 0x7fffffffu) : 0;
+            for (int off = 32; off > 0; off >>= 1) {
+                a += __shfl_xor(a, off, 64);
+                b += __shfl_xor(b, off, 64);
+            }
+            pa += a;
+            pb += b;
+            if (lp < 64) break;
+            kb -= 64;
+        }
+        if (lane == 0) {
+            if (tile > 0)
+                __hip_atomic_store(&st.status[tile], SCAN_PRE | scan_pack(pa + A, pb + B), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            s_pa = pa;
+            s_pb = pb;
+            if (maxa && M > 0) atomicMax(maxa, M);
+        }
+    }
+    __syncthreads();
+    int ea = s_pa + wa + ia - sa, eb = s_pb + wb + ib - sb;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const long long i = base + k;
+        if (i <= n) {
+            if (out_a) out_a[i] = ea;
+            if (out_b) out_b[i] = eb;
+            if (i == n && tot_a) *tot_a = ea;
+        }
+        ea += va[k];
+        eb += vb[k];
+    }
+    // the last workgroup to finish resets the state for the next launch
+    __syncthreads();
+    if (t == 0) {
+        __threadfence();
+        s_last = atomicAdd(st.done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+        for (int k = t; k < (int)gridDim.x; k += SCAN_THREADS) st.status[k] = 0ull;
+        if (t == 0) { *st.ticket = 0u; *st.done = 0u; }
+    }
+}
+
+// Per-cell counts of the single-pass binning: entries in the low and real particles in
+// the high half of each 64-bit counter (also written out as cnt / creal).
+struct ProdCells {
+    const unsigned long long *cnt64;
+    int *cnt, *creal;
+    __device__ void get(long long i, unsigned long long, int &a, int &b) const {
+        const unsigned long long v = cnt64[i];
+        a = (int)(v & 0xffffffffull);
+        b = (int)(v >> 32);
+        cnt[i] = a;
+        creal[i] = b;
+    }
+};
+
+// Wave tasks of every row of cells: ceil(homes of the row / 64).
+template <typename T>
+struct ProdRows {
+    const SysDesc<T> *sys;
+    const int *rowoff;
+    const int *hstart;
+    int nsys;
+    __device__ void get(long long r, unsigned long long, int &a, int &b) const {
+        const int s = last_le(rowoff, 0, nsys + 1, (int)r);
+        const int ncx = sys[s].box.nc[0];
+        const int c0 = sys[s].cell_off + ((int)r - rowoff[s]) * ncx;
+        a = (hstart[c0 + ncx] - hstart[c0] + 63) >> 6;
+        b = 0;
+    }
+};
 
 // ------------------------------------------------------------------------------
 // sweep skeleton
@@ -869,7 +1010,7 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
 // successive requests from successive counters, so the counters advance evenly.  Slots at
 // or beyond the caller's capacity go to a library-owned spill area (a slot past it sets
 // an overflow word).  Partially used chunks and chunks never handed out below the pair
-// count leave holes; k_nl_holes / k_nl_fill move entries from beyond the pair count into
+// count leave holes; k_scan_pair<ProdHoles> / k_nl_fill move entries from beyond the pair count into
 // them, so the final list is dense (its order is unspecified, as in the reference).
 constexpr int NL_CHUNK = 512;
 constexpr int NL_COUNTERS = 64;
@@ -1083,25 +1224,21 @@ struct OpNeighborList {
     }
 };
 
-// Per chunk: holes below the pair count P and entries at or beyond P (movers).
-__global__ void k_nl_holes(long long nchunks, const int *__restrict__ used, const unsigned long long *__restrict__ total,
-                           long long cap, int *__restrict__ holes, int *__restrict__ movers) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > nchunks) return;
-    const long long P = (long long)*total;
-    if (c == nchunks || P > cap) {
-        holes[c] = 0;
-        movers[c] = 0;
-        return;
+// Per chunk of the neighbour-list output: holes below the pair count P (the uniform
+// value) and entries at or beyond P (movers); none when P exceeds the capacity.
+struct ProdHoles {
+    const int *used;
+    long long cap;
+    __device__ void get(long long c, unsigned long long total, int &a, int &b) const {
+        const long long P = (long long)total;
+        a = 0;
+        b = 0;
+        if (P > cap) return;
+        const long long start = c * NL_CHUNK, vend = start + used[c];
+        if (vend < P) a = (int)(min(start + NL_CHUNK, P) - vend);
+        if (vend > P) b = (int)(vend - max(start, P));
     }
-    const long long start = c * NL_CHUNK, u = used[c];
-    const long long vend = start + u;
-    long long h = 0, m = 0;
-    if (vend < P) h = min(start + NL_CHUNK, P) - vend;
-    if (vend > P) m = vend - max(start, P);
-    holes[c] = (int)h;
-    movers[c] = (int)m;
-}
+};
 
 // Largest k in [lo, hi) with a[k] <= v (non-decreasing a).
 __device__ inline long long last_le64(const int *__restrict__ a, long long lo, long long hi, long long v) {
@@ -1113,6 +1250,8 @@ __device__ inline long long last_le64(const int *__restrict__ a, long long lo, l
     return lo;
 }
 
+// Moves the entries at or beyond the pair count P (the movers) into the holes below it:
+// hole k (in the order of hstart) takes mover k (in the order of mstart).
 template <typename T>
 __global__ void k_nl_fill(long long nchunks, const int *__restrict__ used, const int *__restrict__ hstart,
                           const int *__restrict__ mstart, const unsigned long long *__restrict__ total, NLOut<T> out) {
