@@ -20,7 +20,8 @@ CLM_F32, CLM_F64 = 0, 1
 ORTHORHOMBIC, TRICLINIC, NONPERIODIC = 0, 1, 2
 
 EXPORTS = [
-    "clm_create", "clm_destroy", "clm_last_error", "clm_set_stream", "clm_get_stream", "clm_version",
+    "clm_create", "clm_destroy", "clm_last_error", "clm_set_stream", "clm_get_stream", "clm_join_stream",
+    "clm_version",
     "clm_set_box", "clm_get_box", "clm_get_system_box", "clm_box_compute", "clm_set_systems",
     "clm_set_positions", "clm_set_types", "clm_set_velocities", "clm_set_positions_y", "clm_clear_positions_y",
     "clm_set_types_y", "clm_set_velocities_y",
@@ -79,6 +80,7 @@ def lib():
         "clm_last_error": ([P], C.c_char_p),
         "clm_set_stream": ([P, P], I),
         "clm_get_stream": ([P, P], I),
+        "clm_join_stream": ([P, P, I], I),
         "clm_version": ([], C.c_char_p),
         "clm_set_box": ([P, P, I, D, I], I),
         "clm_get_box": ([P, C.POINTER(BoxInfo)], I),

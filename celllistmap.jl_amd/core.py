@@ -171,25 +171,19 @@ class _Context:
     def use_torch_stream(self):
         """Order the library's stream after the work pending on torch's current stream
         (which wrote the tensors the next call reads), and torch's stream after the
-        library's.  The library keeps its own (non-blocking) stream: it never launches
-        on torch's null stream, which graph capture cannot use."""
-        torch = _torch()
-        ext = getattr(self, "_ext", None)
-        if ext is None:
-            sp = C.c_void_p()
-            self.check(self.lib.clm_get_stream(self.ptr, C.byref(sp)), "get_stream")
-            ext = torch.cuda.ExternalStream(sp.value, device=f"cuda:{self.device}")
-            self._ext = ext
-        cur = torch.cuda.current_stream(self.device)
-        ext.wait_stream(cur)
-        cur.wait_stream(ext)
+        library's (clm_join_stream: two cached events, recorded and waited in C).  The
+        library keeps its own (non-blocking) stream: it never launches on torch's null
+        stream, which graph capture cannot use."""
+        s = _torch().cuda.current_stream(self.device).cuda_stream
+        self.check(self.lib.clm_join_stream(self.ptr, C.c_void_p(s), 1), "join_stream")
+        self._joined = True
 
     def torch_after(self):
         """Make torch's current stream wait for work the library left queued (an
         asynchronous device-to-device copy of a tensor torch may overwrite next)."""
-        ext = getattr(self, "_ext", None)
-        if ext is not None:
-            _torch().cuda.current_stream(self.device).wait_stream(ext)
+        if getattr(self, "_joined", False):
+            s = _torch().cuda.current_stream(self.device).cuda_stream
+            self.check(self.lib.clm_join_stream(self.ptr, C.c_void_p(s), 0), "join_stream")
 
 
 def _positions(x, dtype_code: Optional[int], dim: Optional[int] = None):

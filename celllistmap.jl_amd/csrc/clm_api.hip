@@ -88,7 +88,11 @@ struct clm_ctx {
     // operator buffers
     DBuf nl_i, nl_j, nl_d, nl_count, outbuf, acc, hist_c, hist_v, md_d2, md_a, md_b;
     DBuf sp_i, sp_j, sp_d, ch_used, ch_holes, ch_movers, ch_hstart, ch_mstart;
-    uint32_t *h_flags = nullptr;
+    uint32_t *h_flags = nullptr;  // pinned: [0, 4) operator read-backs, [4, 8) deferred binning status
+    // A single-pass build enqueued without reading its status block back (bin_pending):
+    // the next synchronisation reads it with whatever the operator reads back.
+    bool bin_pending = false;
+    bool no_defer = false;  // the next build reads its status back first (after an overflow)
 
     // second particle set (two-set / cross mode): a sub-context on the same stream and
     // box holding the y positions and their cell list
@@ -102,9 +106,16 @@ struct clm_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
     std::map<std::string, std::pair<double, int64_t>> prof_acc;
+
+    // events of clm_join_stream (created on first use, timing off)
+    hipEvent_t join_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
+
+// Internal: the cell list was found invalid after the fact (a deferred build overflowed);
+// the entry point runs again on a rebuilt list.  Never returned through the ABI.
+constexpr clm_status CLM_REDO = (clm_status)15;
 
 clm_status fail(clm_ctx *c, clm_status s, const std::string &msg) {
     if (c) c->err = msg;
@@ -181,11 +192,23 @@ hipError_t timed(clm_ctx *c, const char *name, F &&launch) {
         if (e_ != hipSuccess) return hip_fail(c, e_, name);             \
     } while (0)
 
+clm_status check_flags(clm_ctx *c, uint32_t fl);
+clm_status check_deferred_build(clm_ctx *c);
+
 clm_status sync(clm_ctx *c) {
+    const bool pending = c->bin_pending;
+    if (pending) {
+        hipError_t e = hipMemcpyAsync(c->h_flags + 4, c->flags.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "hipMemcpyAsync");
+    }
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
     clm_ctx *r = c->root ? c->root : c;
     if (r->prof) collect_profile(r);
+    if (pending) {
+        c->bin_pending = false;
+        return check_deferred_build(c);
+    }
     return CLM_OK;
 }
 
@@ -510,6 +533,13 @@ clm_status enqueue_scatter(clm_ctx *c) {
     return enqueue_sort<T>(c, 0, c->e_cap);
 }
 
+// A build may skip its status read-back: a periodic single-set owner context whose
+// bucket size and entry capacity are known, unless the last deferred build overflowed.
+bool can_defer(const clm_ctx *c) {
+    return !c->no_defer && !c->two_set && !c->root && c->celltype != CLM_NONPERIODIC && c->bucket > 0 && c->e_cap > 0 &&
+           c->sweep_variant == 1;
+}
+
 clm_status check_flags(clm_ctx *c, uint32_t fl) {
     if (fl & FLAG_SCAN) return fail(c, CLM_ERR_HIP, "a single-pass scan did not complete (internal error)");
     if (fl & FLAG_NAN) return fail(c, CLM_ERR_NAN, "Invalid coordinates found (NaN or Inf)");
@@ -530,8 +560,36 @@ int bucket_size(const clm_ctx *c, int mx, size_t esize) {
     return (int)b;
 }
 
-// Single-pass binning (k_bin_bucket + unpack + scans + k_cell_sort), one host
-// synchronisation.  done = false when the buckets cannot be sized within the memory
+// Status of a build enqueued without its read-back (the flags block copied by sync()):
+// errors are reported as from a synchronous build; an overflow (a cell past its bucket,
+// or more entries than the list holds) invalidates the list, and CLM_REDO tells the entry
+// point to run again on a list rebuilt with its status read first.  The sweeps skip such
+// a list on the device (SweepArgs::bin_flags), so nothing read it out of range.
+clm_status check_deferred_build(clm_ctx *c) {
+    const uint32_t fl = c->h_flags[4];
+    const int mx = (int)c->h_flags[5], total = (int)c->h_flags[6];
+    clm_status s = check_flags(c, fl);
+    if (s) {
+        c->cl_valid = false;
+        c->order_valid = false;
+        return s;
+    }
+    if ((fl & FLAG_OVERFLOW) || (int64_t)total > c->e_cap) {
+        c->cl_valid = false;
+        c->dirty = true;
+        c->order_valid = false;  // cells past the capacity were not sorted
+        c->no_defer = true;
+        if (fl & FLAG_OVERFLOW) c->bucket = bucket_size(c, mx, c->dtype == CLM_F32 ? sizeof(Ent<float>) : sizeof(Ent<double>));
+        return CLM_REDO;
+    }
+    c->n_entries = total;
+    return CLM_OK;
+}
+
+// Single-pass binning (k_bin_bucket + k_scan_pair + k_cell_sort).  With a bucket size and
+// an entry capacity known from an earlier build, nothing is read back: the sort covers the
+// capacity (it reads the count on the device) and the status is checked at the next
+// synchronisation.  Otherwise one host synchronisation reads the status first.  done = false when the buckets cannot be sized within the memory
 // limit (the caller then bins in two passes).
 template <typename T>
 clm_status bin_bucketed(clm_ctx *c, bool &done) {
@@ -568,6 +626,13 @@ clm_status bin_bucketed(clm_ctx *c, bool &done) {
                                                      dim3(SCAN_THREADS), 0, c->stream, (long long)c->ncells, pc,
                                                      (const unsigned long long *)nullptr, (int *)c->cstart.p, (int *)c->rstart.p, (int *)c->flags.p + 1,
                                                      (int *)c->flags.p + 2, st, (unsigned int *)c->flags.p, FLAG_SCAN));
+        if (can_defer(c)) {
+            if ((s = ensure_split<T>(c))) return s;
+            if ((s = enqueue_sort<T>(c, c->bucket, c->e_cap))) return s;
+            c->bin_pending = true;
+            done = true;
+            return CLM_OK;
+        }
         HIPCHK(c, hipMemcpyAsync(c->h_flags, c->flags.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         if ((s = sync(c))) return s;
         if ((s = check_flags(c, c->h_flags[0]))) return s;
@@ -630,6 +695,7 @@ clm_status update_cell_list_t(clm_ctx *c) {
         if ((s = enqueue_scatter<T>(c))) return s;
         if (c->bucket == 0) c->bucket = bucket_size(c, mx, sizeof(Ent<T>));
     }
+    if (!c->bin_pending) c->no_defer = false;
     c->cl_valid = true;
     c->dirty = false;
     for (bool &m : c->mode_valid) m = false;
@@ -770,6 +836,9 @@ clm_status run_sweep_on(clm_ctx *c, clm_ctx *nb, const char *name, const Op &op,
     sa.en_bytes = (uint32_t)std::min<size_t>(nb->E.bytes, 0xFFFFFFF0u);
     if (c->sweep_variant == 1 && nb->E.bytes > 0xFFFFFFF0u)
         return fail(c, CLM_ERR_ARG, "cell list above 4 GiB: use sweep variant 0");
+    // a build whose status is still to be read: the sweep checks it on the device first
+    sa.bin_flags = c->bin_pending ? (const uint32_t *)c->flags.p : nullptr;
+    sa.e_cap = c->e_cap;
     sa.reals = (const Ent<T> *)c->E_real.p;
     sa.ghosts = (const Ent<T> *)c->E_ghost.p;
     sa.reals_bytes = (uint32_t)std::min<size_t>(c->E_real.bytes, 0xFFFFFFF0u);
@@ -1040,6 +1109,7 @@ clm_status graph_capture_t(clm_ctx *c, const clm_softcore *p, void *forces) {
     if (c->celltype == CLM_NONPERIODIC) return fail(c, CLM_ERR_ARG, "graph capture needs a periodic box");
     if (c->two_set) return fail(c, CLM_ERR_ARG, "graph capture is for one set of particles");
     if ((s = ensure_cell_list(c))) return s;
+    if (c->bin_pending && (s = sync(c))) return s;
     // headroom for fluctuating ghost counts between replays
     const int64_t want = c->n_entries + c->n_entries / 8 + 1024;
     if (c->e_cap < want) {
@@ -1178,6 +1248,16 @@ clm_status mindist_t(clm_ctx *c, int64_t *oi, int64_t *oj, double *od) {
 // C ABI
 // ------------------------------------------------------------------------------
 
+// Runs an entry point, and once more if it found its cell list invalid (CLM_REDO): the
+// second run rebuilds the list with its status read first, so it cannot ask again.
+template <typename F>
+clm_status with_redo(clm_ctx *c, F f) {
+    clm_status s = f();
+    if (s == CLM_REDO) s = f();
+    if (s == CLM_REDO) s = fail(c, CLM_ERR_HIP, "cell list rebuild did not settle");
+    return s;
+}
+
 extern "C" {
 
 const char *clm_version(void) { return "celllistmap-hip 0.1.0 (gfx950)"; }
@@ -1198,7 +1278,7 @@ clm_status clm_create(clm_ctx **out, int device, int dtype, int dim) {
     c->own_stream = true;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) c->num_cus = prop.multiProcessorCount;
-    if (hipHostMalloc((void **)&c->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&c->h_flags, sizeof(uint32_t) * 8, hipHostMallocDefault) != hipSuccess) {
         hipStreamDestroy(c->stream);
         delete c;
         return CLM_ERR_HIP;
@@ -1233,12 +1313,30 @@ clm_status clm_destroy(clm_ctx *c) {
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
     if (c->h_flags) hipHostFree(c->h_flags);
+    for (hipEvent_t e : c->join_ev)
+        if (e) hipEventDestroy(e);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
     return CLM_OK;
 }
 
 const char *clm_last_error(const clm_ctx *c) { return c ? c->err.c_str() : "NULL context"; }
+
+clm_status clm_join_stream(clm_ctx *c, void *other, int both) {
+    if (!c) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    hipStream_t o = (hipStream_t)other;
+    if (o == c->stream) return CLM_OK;
+    for (hipEvent_t &e : c->join_ev)
+        if (!e) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (both) {
+        HIPCHK(c, hipEventRecord(c->join_ev[0], o));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev[0], 0));
+    }
+    HIPCHK(c, hipEventRecord(c->join_ev[1], c->stream));
+    HIPCHK(c, hipStreamWaitEvent(o, c->join_ev[1], 0));
+    return CLM_OK;
+}
 
 clm_status clm_get_stream(clm_ctx *c, void **stream) {
     if (!c || !stream) return CLM_ERR_ARG;
@@ -1454,9 +1552,11 @@ clm_status clm_update_cell_list(clm_ctx *c) {
     hipSetDevice(c->device);
     c->dirty = true;
     if (c->two_set && c->ysub) c->ysub->dirty = true;
-    clm_status s = c->two_set ? ensure_cross(c) : ensure_cell_list(c);
-    if (s) return s;
-    return sync(c);
+    return with_redo(c, [&] {
+        clm_status s = c->two_set ? ensure_cross(c) : ensure_cell_list(c);
+        if (s) return s;
+        return sync(c);
+    });
 }
 
 // Second set of particles (xpositions/ypositions of ParticleSystem, ParticleSystem.jl:149-214).
@@ -1474,7 +1574,7 @@ clm_status clm_set_positions_y(clm_ctx *c, const void *y, int64_t n, int on_devi
         y2->stream = c->stream;
         y2->own_stream = false;
         y2->root = c;
-        if (hipHostMalloc((void **)&y2->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc((void **)&y2->h_flags, sizeof(uint32_t) * 8, hipHostMallocDefault) != hipSuccess) {
             delete y2;
             return fail(c, CLM_ERR_HIP, "hipHostMalloc");
         }
@@ -1509,9 +1609,7 @@ clm_status clm_set_velocities_y(clm_ctx *c, const void *v, int64_t n, int on_dev
     return s ? fail(c, s, c->ysub->err) : CLM_OK;
 }
 
-clm_status clm_cell_list_info(clm_ctx *c, int64_t *out4) {
-    if (!c || !out4) return CLM_ERR_ARG;
-    hipSetDevice(c->device);
+clm_status cell_list_info(clm_ctx *c, int64_t *out4) {
     clm_status s = ensure_cell_list(c);
     if (s) return s;
     std::vector<int> cr(c->ncells), cst(c->ncells + 1);
@@ -1530,47 +1628,61 @@ clm_status clm_cell_list_info(clm_ctx *c, int64_t *out4) {
     return CLM_OK;
 }
 
+clm_status clm_cell_list_info(clm_ctx *c, int64_t *out4) {
+    if (!c || !out4) return CLM_ERR_ARG;
+    hipSetDevice(c->device);
+    return with_redo(c, [&] { return cell_list_info(c, out4); });
+}
+
 clm_status clm_neighborlist(clm_ctx *c, int32_t *i, int32_t *j, void *d, int64_t capacity, int64_t *npairs, int on_device) {
     if (!c || !npairs || capacity < 0) return CLM_ERR_ARG;
     if (capacity > 0 && (!i || !j || !d)) return fail(c, CLM_ERR_ARG, "NULL output arrays");
     hipSetDevice(c->device);
-    return c->dtype == CLM_F32 ? neighborlist_t<float>(c, i, j, d, capacity, npairs, on_device)
-                               : neighborlist_t<double>(c, i, j, d, capacity, npairs, on_device);
+    return with_redo(c, [&] {
+        return c->dtype == CLM_F32 ? neighborlist_t<float>(c, i, j, d, capacity, npairs, on_device)
+                                   : neighborlist_t<double>(c, i, j, d, capacity, npairs, on_device);
+    });
 }
 
 clm_status clm_energy_forces(clm_ctx *c, const clm_softcore *p, double *energy, void *forces, int on_device, int64_t *npairs) {
     if (!c) return CLM_ERR_ARG;
     hipSetDevice(c->device);
     if (npairs) *npairs = -1;
-    if (forces)
-        return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energy, 0, forces, nullptr, nullptr, on_device)
-                                   : softcore_t<double, true, false>(c, p, energy, 0, forces, nullptr, nullptr, on_device);
-    return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energy, 0, nullptr, nullptr, nullptr, on_device)
-                               : softcore_t<double, false, false>(c, p, energy, 0, nullptr, nullptr, nullptr, on_device);
+    return with_redo(c, [&] {
+        if (forces)
+            return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energy, 0, forces, nullptr, nullptr, on_device)
+                                       : softcore_t<double, true, false>(c, p, energy, 0, forces, nullptr, nullptr, on_device);
+        return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energy, 0, nullptr, nullptr, nullptr, on_device)
+                                   : softcore_t<double, false, false>(c, p, energy, 0, nullptr, nullptr, nullptr, on_device);
+    });
 }
 
 clm_status clm_energy_forces_batch(clm_ctx *c, const clm_softcore *p, double *energies, void *forces, int on_device) {
     if (!c || !energies) return CLM_ERR_ARG;
     hipSetDevice(c->device);
-    if (forces)
-        return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energies, 1, forces, nullptr, nullptr, on_device)
-                                   : softcore_t<double, true, false>(c, p, energies, 1, forces, nullptr, nullptr, on_device);
-    return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energies, 1, nullptr, nullptr, nullptr, on_device)
-                               : softcore_t<double, false, false>(c, p, energies, 1, nullptr, nullptr, nullptr, on_device);
+    return with_redo(c, [&] {
+        if (forces)
+            return c->dtype == CLM_F32 ? softcore_t<float, true, false>(c, p, energies, 1, forces, nullptr, nullptr, on_device)
+                                       : softcore_t<double, true, false>(c, p, energies, 1, forces, nullptr, nullptr, on_device);
+        return c->dtype == CLM_F32 ? softcore_t<float, false, false>(c, p, energies, 1, nullptr, nullptr, nullptr, on_device)
+                                   : softcore_t<double, false, false>(c, p, energies, 1, nullptr, nullptr, nullptr, on_device);
+    });
 }
 
 clm_status clm_energy_adjoint(clm_ctx *c, const clm_softcore *p, double *energy, void *dedx, double *deps, double *dsig,
                               int on_device) {
     if (!c) return CLM_ERR_ARG;
     hipSetDevice(c->device);
-    return c->dtype == CLM_F32 ? softcore_t<float, true, true>(c, p, energy, 0, dedx, deps, dsig, on_device)
-                               : softcore_t<double, true, true>(c, p, energy, 0, dedx, deps, dsig, on_device);
+    return with_redo(c, [&] {
+        return c->dtype == CLM_F32 ? softcore_t<float, true, true>(c, p, energy, 0, dedx, deps, dsig, on_device)
+                                   : softcore_t<double, true, true>(c, p, energy, 0, dedx, deps, dsig, on_device);
+    });
 }
 
 clm_status clm_graph_capture_energy_forces(clm_ctx *c, const clm_softcore *p, void *forces) {
     if (!c) return CLM_ERR_ARG;
     hipSetDevice(c->device);
-    return c->dtype == CLM_F32 ? graph_capture_t<float>(c, p, forces) : graph_capture_t<double>(c, p, forces);
+    return with_redo(c, [&] { return c->dtype == CLM_F32 ? graph_capture_t<float>(c, p, forces) : graph_capture_t<double>(c, p, forces); });
 }
 
 clm_status clm_graph_launch(clm_ctx *c, double *energies) {
@@ -1582,14 +1694,16 @@ clm_status clm_graph_launch(clm_ctx *c, double *energies) {
 clm_status clm_histogram(clm_ctx *c, int nbins, double width, int64_t *counts, double *vsum) {
     if (!c || !counts) return CLM_ERR_ARG;
     hipSetDevice(c->device);
-    return c->dtype == CLM_F32 ? histogram_t<float>(c, nbins, width, counts, vsum)
-                               : histogram_t<double>(c, nbins, width, counts, vsum);
+    return with_redo(c, [&] {
+        return c->dtype == CLM_F32 ? histogram_t<float>(c, nbins, width, counts, vsum)
+                                   : histogram_t<double>(c, nbins, width, counts, vsum);
+    });
 }
 
 clm_status clm_min_distance(clm_ctx *c, int64_t *i, int64_t *j, double *d) {
     if (!c || !i || !j || !d) return CLM_ERR_ARG;
     hipSetDevice(c->device);
-    return c->dtype == CLM_F32 ? mindist_t<float>(c, i, j, d) : mindist_t<double>(c, i, j, d);
+    return with_redo(c, [&] { return c->dtype == CLM_F32 ? mindist_t<float>(c, i, j, d) : mindist_t<double>(c, i, j, d); });
 }
 
 clm_status clm_profile_enable(clm_ctx *c, int enable) {

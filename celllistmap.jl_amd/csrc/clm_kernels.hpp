@@ -332,8 +332,12 @@ __global__ void __launch_bounds__(256) k_cell_sort(int ncells, const int *__rest
     if (k >= cstart[ncells]) return;
     const int c = last_le(cstart, 0, ncells, k);  // the last cell starting at or before k: never empty
     const int s0 = cstart[c];
-    const int n = cstart[c + 1] - s0;
+    int n = cstart[c + 1] - s0;
     if ((int64_t)s0 + n > cap) return;
+    if (bucket > 0 && n > bucket) {  // the cell outgrew its bucket (the build is redone): stay inside it
+        n = bucket;
+        if (k - s0 >= n) return;
+    }
     const Ent<T> *src = Ein + (bucket > 0 ? (size_t)c * bucket : (size_t)s0);
     const Ent<T> e = src[k - s0];
     const uint32_t we = ent_w(e);
@@ -584,7 +588,19 @@ struct SweepArgs {
     const Ent<T> *reals;
     const Ent<T> *ghosts;
     uint32_t reals_bytes, ghosts_bytes;
+    // status block of a build not yet checked on the host (null when checked): a list
+    // with an error or an overflow is not walked (the host rebuilds it and runs again)
+    const uint32_t *bin_flags;
+    long long e_cap;
 };
+
+// Whether the cell list of sa may be walked (see SweepArgs::bin_flags).
+template <typename T>
+__device__ inline bool list_usable(const SweepArgs<T> &sa) {
+    if (!sa.bin_flags) return true;
+    const uint32_t f = sa.bin_flags[0];
+    return !(f & (FLAG_OVERFLOW | FLAG_NAN | FLAG_RANGE | FLAG_SCAN)) && (long long)sa.bin_flags[2] <= sa.e_cap;
+}
 
 // Dynamic task fetch: blocks are dealt to XCDs round-robin (block b runs on XCD b % 8),
 // so each XCD walks its own contiguous share of the tasks first (neighbouring rows share
@@ -616,6 +632,7 @@ __global__ void __launch_bounds__(64) k_sweep(const Ent<T> *__restrict__ E, cons
                                               const SweepArgs<T> sa, const Op op) {
     __shared__ typename Op::Shared sh;
     __shared__ int s_task;
+    if (!list_usable(sa)) return;
     const int lane = threadIdx.x;
     constexpr int B = SweepBatch<T>::value;
     typename Op::Wave ws;
@@ -800,6 +817,7 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                                                   const SweepArgs<T> sa, const Op op) {
     __shared__ typename Op::Shared sh;
     __shared__ int s_task;
+    if (!list_usable(sa)) return;
     const int lane = threadIdx.x;
 #ifndef CLM_SWEEP_U
 #define CLM_SWEEP_U 4

@@ -83,6 +83,11 @@ const char *clm_last_error(const clm_ctx *ctx);
 clm_status clm_set_stream(clm_ctx *ctx, void *stream);
 /* The stream the context launches on (to order a caller's own device work against it). */
 clm_status clm_get_stream(clm_ctx *ctx, void **stream);
+/* Orders the context's stream and `other` (a hipStream_t passed as void*): with both = 1,
+ * work queued on either stream after the call starts after the work queued on both
+ * before it; with both = 0 only `other` waits for the context's earlier work.  No
+ * reference counterpart: plumbing for callers whose buffers another stream writes. */
+clm_status clm_join_stream(clm_ctx *ctx, void *other, int both);
 /* Library version string. */
 const char *clm_version(void);
 

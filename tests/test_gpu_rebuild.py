@@ -58,3 +58,52 @@ def test_rebuild_energy_forces_batch(clm, O):
             assert abs(e[k] - ref["energy"]) <= 1e-10 * abs(ref["energy"])
         xs = [x + rng.normal(0, 0.2, x.shape) for x in xs]
         b.set_positions(np.concatenate(xs))
+
+
+@pytest.mark.parametrize("tric", [False, True])
+def test_rebuild_past_entry_capacity(clm, O, tric):
+    """Later builds bin in one pass without reading their status back; one that outgrows
+    the entry capacity sized by the earlier builds (every particle on an edge of the box
+    has three images) is found at the operator's read-back and rebuilt.  The first
+    operator after each such update -- histogram, energy, neighbour list -- matches the
+    oracle."""
+    say(f"[rebuild] past the entry capacity, tric={tric}")
+    dtype = np.float32
+    rng = np.random.default_rng(5 + tric)
+    M = np.array([[23.0, 4.0, 2.0], [0.0, 23.0, 3.0], [0.0, 0.0, 23.0]]) if tric else np.diag([23.0] * 3)
+    uc = M if tric else np.diag(M)
+    rc = 3.5
+    box = O.make_box(uc, rc, lcell=2, dtype=dtype)
+    x = (M @ rng.random((3, 2000))).T.astype(dtype)
+    sys_ = clm.ParticleSystem(xpositions=x, unitcell=uc, cutoff=rc, lcell=2, output=clm.NeighborListOp())
+    check(clm, O, sys_, box, x)                        # two-pass build
+    x = (x + rng.normal(0, 0.2, x.shape)).astype(dtype)
+    clm.update_(sys_, xpositions=x)
+    check(clm, O, sys_, box, x)                        # one pass, status read at the end
+    n1 = sys_.cell_list_info()["n_particles"]
+
+    def edge():
+        f = rng.random((2000, 3))
+        f[:, 1] *= 0.15
+        f[:, 2] *= 0.15
+        return (M @ f.T).T.astype(dtype)
+
+    x2 = edge()
+    clm.update_(sys_, xpositions=x2)
+    counts = clm.pairwise_(clm.DistanceHistogram(20, 0.2), sys_)
+    ref, _ = O.histogram(box, x2, 20, 0.2, mode=1)
+    assert np.array_equal(counts, ref)
+    assert sys_.cell_list_info()["n_particles"] > 1.3 * n1  # the capacity was outgrown
+
+    x3 = edge()
+    clm.update_(sys_, xpositions=x3)
+    e = clm.pairwise_(clm.SoftCoreEnergyForces(0.2, 3.0), sys_)
+    ref = O.softcore(O.make_box(uc, rc, dtype=np.float64), x3.astype(np.float64), 0.2, 3.0)
+    assert abs(e.energy - ref["energy"]) / abs(ref["energy"]) < 1e-4
+    assert np.linalg.norm(e.forces - ref["forces"]) / np.linalg.norm(ref["forces"]) < 1e-4
+
+    x4 = edge()
+    clm.update_(sys_, xpositions=x4)
+    check(clm, O, sys_, box, x4)
+    clm.update_(sys_, xpositions=x)
+    check(clm, O, sys_, box, x)
