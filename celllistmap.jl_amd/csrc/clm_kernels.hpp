@@ -328,9 +328,41 @@ __global__ void __launch_bounds__(256) k_cell_sort(int ncells, const int *__rest
                                                    int bucket, int64_t cap, Ent<T> *__restrict__ Eout,
                                                    int *__restrict__ real_pos, int *__restrict__ order,
                                                    Ent<T> *__restrict__ Rout, Ent<T> *__restrict__ Gout) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= cstart[ncells]) return;
-    const int c = last_le(cstart, 0, ncells, k);  // the last cell starting at or before k: never empty
+    // The workgroup's entries belong to the cells [c0, c1]; their keys (x, w) are staged
+    // in LDS once when they fit, and every thread ranks its entry against its cell there.
+    constexpr int STAGE = 1024;
+    __shared__ T s_x[STAGE];
+    __shared__ uint32_t s_w[STAGE];
+    __shared__ int s_c0, s_c1;
+    const int total = cstart[ncells];
+    const int kb = blockIdx.x * blockDim.x;
+    if (kb >= total) return;  // the whole workgroup
+    if (threadIdx.x == 0) {
+        const int c0 = last_le(cstart, 0, ncells, kb);
+        s_c0 = c0;
+        s_c1 = last_le(cstart, c0, ncells, min(kb + (int)blockDim.x, total) - 1);
+    }
+    __syncthreads();
+    const int c0 = s_c0, c1 = s_c1;
+    const int base = cstart[c0];
+    const bool staged = cstart[c1 + 1] - base <= STAGE;
+    if (staged) {
+        for (int t = threadIdx.x; t < cstart[c1 + 1] - base; t += blockDim.x) {
+            const int kk = base + t;
+            const int cc = last_le(cstart, c0, c1 + 1, kk);
+            const int s0c = cstart[cc];
+            const int nc = cstart[cc + 1] - s0c;
+            // the same limits as the ranking below (capacity, bucket)
+            if ((int64_t)s0c + nc > cap || (bucket > 0 && kk - s0c >= bucket)) continue;
+            const Ent<T> f = Ein[bucket > 0 ? (size_t)cc * bucket + (kk - s0c) : (size_t)kk];
+            s_x[t] = f.x;
+            s_w[t] = ent_w(f);
+        }
+    }
+    __syncthreads();
+    const int k = kb + threadIdx.x;
+    if (k >= total) return;
+    const int c = last_le(cstart, c0, c1 + 1, k);  // the last cell starting at or before k: never empty
     const int s0 = cstart[c];
     int n = cstart[c + 1] - s0;
     if ((int64_t)s0 + n > cap) return;
@@ -342,12 +374,23 @@ __global__ void __launch_bounds__(256) k_cell_sort(int ncells, const int *__rest
     const Ent<T> e = src[k - s0];
     const uint32_t we = ent_w(e);
     int rank = 0, rrank = 0;
-    for (int m = 0; m < n; ++m) {
-        const Ent<T> f = src[m];
-        const uint32_t wf = ent_w(f);
-        const bool less = (f.x < e.x) || (f.x == e.x && wf < we);
-        rank += less ? 1 : 0;
-        rrank += (less && !w_ghost(wf)) ? 1 : 0;
+    if (staged) {
+        const int o = s0 - base;
+        for (int m = 0; m < n; ++m) {
+            const T fx = s_x[o + m];
+            const uint32_t wf = s_w[o + m];
+            const bool less = (fx < e.x) || (fx == e.x && wf < we);
+            rank += less ? 1 : 0;
+            rrank += (less && !w_ghost(wf)) ? 1 : 0;
+        }
+    } else {
+        for (int m = 0; m < n; ++m) {
+            const Ent<T> f = src[m];
+            const uint32_t wf = ent_w(f);
+            const bool less = (f.x < e.x) || (f.x == e.x && wf < we);
+            rank += less ? 1 : 0;
+            rrank += (less && !w_ghost(wf)) ? 1 : 0;
+        }
     }
     Eout[s0 + rank] = e;
     const int r0 = rstart[c];
