@@ -83,6 +83,7 @@ struct clm_ctx {
     DBuf hc, ntask, hstart[3], tstart[3];
     bool mode_valid[3] = {false, false, false};
     DBuf counters, scan_tmp;
+    bool counters_zeroed = false;  // the caller zeroed the sweep's task counters with its own fill
     DBuf scan_st[3];  // state of the single-pass scans: cells, rows, neighbour-list chunks
 
     // operator buffers
@@ -826,7 +827,8 @@ template <typename T, int MODE, typename Op>
 clm_status run_sweep_on(clm_ctx *c, clm_ctx *nb, const char *name, const Op &op, int grid) {
     HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
     clm_status s;
-    if ((s = fill_int(c, c->counters.p, 16, 0))) return s;
+    if (!c->counters_zeroed && (s = fill_int(c, c->counters.p, 16, 0))) return s;
+    c->counters_zeroed = false;
     SweepArgs<T> sa = sweep_args<T>(c, MODE);
     if (grid <= 0) grid = sweep_occupancy<T, MODE, Op>(c);
     const Ent<T> *E = (const Ent<T> *)c->E.p;
@@ -938,7 +940,10 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         HIPCHK(c, ensure(c->sp_d, sizeof(T) * scap));
         for (DBuf *b : {&c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart})
             HIPCHK(c, ensure(*b, sizeof(int) * (nchunks + 1)));
-        if ((s = fill_zero(c, {{c->nl_count.p, 2 * (int64_t)(nctr + 2)}, {c->ch_used.p, nchunks + 1}}))) return s;
+        HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
+        if ((s = fill_zero(c, {{c->nl_count.p, 2 * (int64_t)(nctr + 2)}, {c->ch_used.p, nchunks + 1}, {c->counters.p, 16}})))
+            return s;
+        c->counters_zeroed = true;  // the sweep below skips its own fill
         out.oi = di; out.oj = dj; out.od = dd;
         out.si = (int32_t *)c->sp_i.p; out.sj = (int32_t *)c->sp_j.p; out.sd = (T *)c->sp_d.p;
         out.cap = cap;
@@ -952,7 +957,9 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         op.chunk_used = (int *)c->ch_used.p;
         op.max_chunks = nchunks;
         op.ordered = mode_once(c) == MODE_A_TRIC;
-        if ((s = run_mode_a<T>(c, "k_sweep_neighborlist", op, grid))) return s;
+        s = run_mode_a<T>(c, "k_sweep_neighborlist", op, grid);
+        c->counters_zeroed = false;
+        if (s) return s;
         // compaction (a no-op on the device when the pairs exceed the capacity), then one
         // host synchronisation for the count and the overflow word
         ScanState st;
