@@ -683,14 +683,22 @@ def _neighborlist(sys: ParticleSystem, capacity, device_output: bool):
         est = npairs_all * sph / max(vol, 1e-300)
         capacity = int(est * 1.1) + 1024
     while True:
-        if device_output:
-            torch = _torch()
-            i = torch.empty(capacity, dtype=torch.int32, device=f"cuda:{ctx.device}")
-            j = torch.empty(capacity, dtype=torch.int32, device=f"cuda:{ctx.device}")
+        # the output arrays are kept and reused while the capacity holds, as the
+        # reference's in-place list is (the previous result is overwritten)
+        bufs = getattr(sys, "_nl_bufs", None)
+        if bufs is not None and bufs[0] == capacity and bufs[1] == bool(device_output):
+            i, j, d = bufs[2:]
         else:
-            i = np.empty(capacity, np.int32)
-            j = np.empty(capacity, np.int32)
-        d = _out_array(device_output, (capacity,), sys.dtype, ctx.device)
+            sys._nl_bufs = None
+            if device_output:
+                torch = _torch()
+                i = torch.empty(capacity, dtype=torch.int32, device=f"cuda:{ctx.device}")
+                j = torch.empty(capacity, dtype=torch.int32, device=f"cuda:{ctx.device}")
+            else:
+                i = np.empty(capacity, np.int32)
+                j = np.empty(capacity, np.int32)
+            d = _out_array(device_output, (capacity,), sys.dtype, ctx.device)
+            sys._nl_bufs = (capacity, bool(device_output), i, j, d)
         npairs = C.c_int64(0)
         rc = L.clm_neighborlist(ctx.ptr, _ptr(i), _ptr(j), _ptr(d), int(capacity), C.byref(npairs), int(device_output))
         if rc == _lib.CLM_ERR_CAPACITY:
