@@ -107,3 +107,35 @@ def test_rebuild_past_entry_capacity(clm, O, tric):
     check(clm, O, sys_, box, x4)
     clm.update_(sys_, xpositions=x)
     check(clm, O, sys_, box, x)
+
+
+@pytest.mark.parametrize("tric", [False, True])
+def test_later_build_with_nan(clm, O, tric):
+    """A NaN coordinate in a later single-pass build (status read with the operator's
+    results) raises the same ArgumentError as in a first build, and the system recovers
+    on the next valid update."""
+    say(f"[rebuild] NaN in a later build, tric={tric}")
+    rng = np.random.default_rng(17 + tric)
+    M = np.array([[20.0, 3.0, 2.0], [0.0, 21.0, 4.0], [0.0, 0.0, 22.0]]) if tric else np.diag([20.0, 21.0, 22.0])
+    uc = M if tric else np.diag(M)
+    rc = 3.0
+    x = (M @ rng.random((3, 1500))).T
+    sys_ = clm.ParticleSystem(xpositions=x, unitcell=uc, cutoff=rc, lcell=2, output=clm.NeighborListOp())
+    box = O.make_box(uc, rc, lcell=2)
+    check(clm, O, sys_, box, x)
+    x = x + rng.normal(0, 0.1, x.shape)
+    clm.update_(sys_, xpositions=x)
+    check(clm, O, sys_, box, x)
+    bad = x.copy()
+    bad[123, 1] = np.nan
+    clm.update_(sys_, xpositions=bad)
+    with pytest.raises(clm.ArgumentError):
+        clm.pairwise_(None, sys_)
+    with pytest.raises(clm.ArgumentError):
+        clm.pairwise_(clm.SoftCoreEnergyForces(0.2, 3.0), sys_)
+    x = x + rng.normal(0, 0.1, x.shape)
+    clm.update_(sys_, xpositions=x)
+    check(clm, O, sys_, box, x)
+    e = clm.pairwise_(clm.SoftCoreEnergyForces(0.2, 3.0), sys_)
+    ref = O.softcore(box, x, 0.2, 3.0)
+    assert abs(e.energy - ref["energy"]) <= 1e-10 * abs(ref["energy"])
