@@ -80,7 +80,7 @@ struct clm_ctx {
     bool split_valid = false;
     int bucket = 0;  // entries per cell bucket of the single-pass binning; 0 = not sized yet, -1 = off
     int64_t e_cap = 0;
-    DBuf hc, ntask, hstart[3], tstart[3];
+    DBuf hc, hstart[3], tstart[3];
     bool mode_valid[3] = {false, false, false};
     DBuf counters, scan_tmp;
     bool counters_zeroed = false;  // the caller zeroed the sweep's task counters with its own fill
@@ -88,7 +88,7 @@ struct clm_ctx {
 
     // operator buffers
     DBuf nl_i, nl_j, nl_d, nl_count, outbuf, acc, hist_c, hist_v, md_d2, md_a, md_b;
-    DBuf sp_i, sp_j, sp_d, ch_used, ch_holes, ch_movers, ch_hstart, ch_mstart;
+    DBuf sp_i, sp_j, sp_d, ch_used, ch_hstart, ch_mstart;
     uint32_t *h_flags = nullptr;  // pinned: [0, 4) operator read-backs, [4, 8) deferred binning status
     // A single-pass build enqueued without reading its status block back (bin_pending):
     // the next synchronisation reads it with whatever the operator reads back.
@@ -757,7 +757,6 @@ template <typename T>
 clm_status enqueue_mode(clm_ctx *c, int mode) {
     mode = mode_table(mode);
     const int nc1 = c->ncells + 1;
-    HIPCHK(c, ensure(c->ntask, sizeof(int) * (c->nrows + 1)));
     HIPCHK(c, ensure(c->tstart[mode], sizeof(int) * (c->nrows + 1)));
     clm_status s;
     if (mode_all_entries(mode)) {
@@ -938,7 +937,7 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         HIPCHK(c, ensure(c->sp_i, sizeof(int32_t) * scap));
         HIPCHK(c, ensure(c->sp_j, sizeof(int32_t) * scap));
         HIPCHK(c, ensure(c->sp_d, sizeof(T) * scap));
-        for (DBuf *b : {&c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart})
+        for (DBuf *b : {&c->ch_used, &c->ch_hstart, &c->ch_mstart})
             HIPCHK(c, ensure(*b, sizeof(int) * (nchunks + 1)));
         HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
         if ((s = fill_zero(c, {{c->nl_count.p, 2 * (int64_t)(nctr + 2)}, {c->ch_used.p, nchunks + 1}, {c->counters.p, 16}})))
@@ -1129,7 +1128,6 @@ clm_status graph_capture_t(clm_ctx *c, const clm_softcore *p, void *forces) {
     HIPCHK(c, ensure(c->outbuf, sizeof(T) * nD));
     HIPCHK(c, ensure(c->hc, sizeof(int) * (c->ncells + 1)));
     HIPCHK(c, ensure(c->hstart[MODE_B], sizeof(int) * (c->ncells + 1)));
-    HIPCHK(c, ensure(c->ntask, sizeof(int) * (c->nrows + 1)));
     HIPCHK(c, ensure(c->tstart[MODE_B], sizeof(int) * (c->nrows + 1)));
     HIPCHK(c, ensure(c->counters, sizeof(int) * 16));
     HIPCHK(c, ensure(c->acc, sizeof(double) * (c->nsys + 16)));
@@ -1306,11 +1304,11 @@ clm_status clm_destroy(clm_ctx *c) {
         c->ysub = nullptr;
     }
     DBuf *bufs[] = {&c->pos, &c->types, &c->vel, &c->xw, &c->home_cell, &c->cnt, &c->creal, &c->cstart,
-                    &c->cur_real, &c->cur_ghost, &c->E, &c->flags, &c->hc, &c->ntask, &c->hstart[0],
+                    &c->cur_real, &c->cur_ghost, &c->E, &c->flags, &c->hc, &c->hstart[0],
                     &c->hstart[1], &c->hstart[2], &c->tstart[0], &c->tstart[1], &c->tstart[2], &c->counters,
                     &c->scan_tmp, &c->nl_i, &c->nl_j, &c->nl_d, &c->nl_count, &c->outbuf, &c->acc,
                     &c->hist_c, &c->hist_v, &c->md_d2, &c->md_a, &c->md_b, &c->sp_i, &c->sp_j, &c->sp_d,
-                    &c->ch_used, &c->ch_holes, &c->ch_movers, &c->ch_hstart, &c->ch_mstart, &c->sys_desc,
+                    &c->ch_used, &c->ch_hstart, &c->ch_mstart, &c->sys_desc,
                     &c->sys_of, &c->rowoff, &c->E_raw, &c->rstart, &c->real_pos, &c->cnt64, &c->maxcnt, &c->order,
                     &c->E_real, &c->E_ghost, &c->scan_st[0], &c->scan_st[1], &c->scan_st[2]};
     for (DBuf *b : bufs)

@@ -916,6 +916,16 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
             for (int dy = -L; dy <= L; ++dy) {
                 const int base = sd.cell_off + ((cz + dz) * ncy + (cy + dy)) * ncx;
                 const bool selfrow = (dz == 0 && dy == 0);
+                // SPLIT: real entries only in the rows from the own row on; ghost entries
+                // only where the row holds ghost images within the wave's reach (rows with
+                // neither are skipped before any window arithmetic)
+                const bool fwd = dz > 0 || (dz == 0 && dy >= 0);
+                bool ghost_row = false;
+                if (SPLIT) {
+                    const int g0 = base + max(cxlo - L, 0), g1 = base + min(cxhi + L + 1, ncx);
+                    ghost_row = uniform_int((csn[g1] - sa.hstart[g1]) - (csn[g0] - sa.hstart[g0])) > 0;
+                    if (!fwd && !ghost_row) continue;
+                }
                 const T gy = slab_gap(bx, 1, cy + dy, p.y, my);
                 const T r2 = rc2 - gy * gy - gz * gz;
                 // the lane's window: cells [ka, kb] of the row (before the stencil limits)
@@ -1041,12 +1051,9 @@ __global__ void __launch_bounds__(64) k_sweep_win(const Ent<T> *__restrict__ E, 
                 if (SPLIT) {
                     // each pair of real particles from the one earlier in the cell order;
                     // each pair with a ghost image from the home of the lower index (the
-                    // reference's rule), over the full stencil -- skipped in rows without
-                    // ghost images within the wave's reach
-                    if (dz > 0 || (dz == 0 && dy >= 0)) walk(std::integral_constant<int, 0>{});
-                    const int g0 = base + max(cxlo - L, 0), g1 = base + min(cxhi + L + 1, ncx);
-                    const int ng = uniform_int((csn[g1] - sa.hstart[g1]) - (csn[g0] - sa.hstart[g0]));
-                    if (ng > 0) walk(std::integral_constant<int, 1>{});
+                    // reference's rule), over the full stencil
+                    if (fwd) walk(std::integral_constant<int, 0>{});
+                    if (ghost_row) walk(std::integral_constant<int, 1>{});
                 } else {
                     walk(std::integral_constant<int, 0>{});
                 }
