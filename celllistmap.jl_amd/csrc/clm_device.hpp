@@ -72,6 +72,20 @@ __device__ inline float dev_sqrt_rn<float>(float v) { return __builtin_sqrtf(v);
 template <>
 __device__ inline double dev_sqrt_rn<double>(double v) { return __builtin_sqrt(v); }
 
+// Correctly rounded square root of a float x >= 2^-96: the hardware estimate (within an
+// ulp) corrected by one step to either side -- the candidate whose residual x - s*s'
+// (one fma) changes sign.  For such x this is the full routine's result; the full
+// routine also scales smaller arguments and passes zero, infinities and NaN through.
+__device__ inline float sqrt_rn_above_tiny(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const int si = __builtin_bit_cast(int, s);
+    const float sm = __builtin_bit_cast(float, si - 1), sp = __builtin_bit_cast(float, si + 1);
+    float r = fmaf(-sm, s, x) <= 0.0f ? sm : s;
+    r = fmaf(-sp, s, x) > 0.0f ? sp : r;
+    return r;
+}
+constexpr float SQRT_FAST_MIN = 0x1p-96f;
+
 template <typename T>
 __device__ inline T dev_div_rn(T a, T b) { return a / b; }
 

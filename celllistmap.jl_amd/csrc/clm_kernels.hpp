@@ -1138,9 +1138,17 @@ struct NLOut {
                 b = max(a, b);
                 a = lo;
             }
+            const T d2 = t < m ? ld2[t] : T(1);
+            T d;
+            if constexpr (sizeof(T) == 4) {  // the short form unless a lane's d2 is tiny (or 0)
+                if (ballot64(!(d2 >= SQRT_FAST_MIN)) == 0) d = sqrt_rn_above_tiny(d2);
+                else d = dev_sqrt_rn(d2);
+            } else {
+                d = dev_sqrt_rn(d2);
+            }
             __builtin_nontemporal_store(a, pi + t);
             __builtin_nontemporal_store(b, pj + t);
-            __builtin_nontemporal_store(dev_sqrt_rn(ld2[t]), pd + t);
+            __builtin_nontemporal_store(d, pd + t);
         }
     }
     __device__ void get(long long s, int &i, int &j, T &d) const {
