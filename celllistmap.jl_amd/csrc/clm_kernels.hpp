@@ -1106,8 +1106,8 @@ struct NLOut {
             atomicOr(overflow, 1u);
         }
     }
-    // The wave's staged pairs li/lj/ld2[0, m) into the consecutive slots [s, s + m) (s
-    // and m wave-uniform): the run lies in the caller's arrays or in the spill area as a
+    // The wave's staged pairs li/lj/ld2[0, m) (li: indices, lj: entry words) into the
+    // consecutive slots [s, s + m) (s and m wave-uniform): the run lies in the caller's arrays or in the spill area as a
     // whole, so the destination is chosen once per run; only a run across the end of
     // either takes the per-slot path.  ordered: each pair as (lower index, higher index).
     // The list is far larger than the caches and not read again by the sweep, so it is
@@ -1121,7 +1121,7 @@ struct NLOut {
             pi = si + (s - cap); pj = sj + (s - cap); pd = sd + (s - cap);
         } else {
             for (int t = threadIdx.x; t < m; t += 64) {
-                int a = li[t], b = lj[t];
+                int a = li[t], b = (int)w_idx((uint32_t)lj[t]);
                 if (ordered) {
                     const int lo = min(a, b);
                     b = max(a, b);
@@ -1132,7 +1132,7 @@ struct NLOut {
             return;
         }
         for (int t = threadIdx.x; t < m; t += 64) {
-            int a = li[t], b = lj[t];
+            int a = li[t], b = (int)w_idx((uint32_t)lj[t]);
             if (ordered) {
                 const int lo = min(a, b);
                 b = max(a, b);
@@ -1278,7 +1278,7 @@ struct OpNeighborList {
         if (ok) {
             const int k = l.n + rank_in(m);
             sh.i[k] = (int)w_idx(ent_w(p));
-            sh.j[k] = (int)w_idx(ent_w(q));
+            sh.j[k] = (int)ent_w(q);  // the entry word: its index is taken at the flush
             sh.d2[k] = d2;
         }
         l.n += __popcll(m);
