@@ -93,3 +93,33 @@ def test_python_box_info_documented(clm):
     assert clm.box_info([10, 10, 10], 1.0)["nc"] == [13, 13, 13]
     with pytest.raises(clm.ArgumentError):
         clm.box_info([10, 10, 10], 5.0)
+
+
+def test_struct_layouts_match_header(clm, tmp_path):
+    """The ctypes mirrors of clm_softcore and clm_box_info have the C layout: a few lines
+    of C compiled against include/clm_hip.h print sizes and field offsets."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text("""#include <stddef.h>
+#include <stdio.h>
+#include "clm_hip.h"
+int main(void) {
+    printf("%zu %zu %zu %zu\\n", sizeof(clm_softcore), offsetof(clm_softcore, eps), offsetof(clm_softcore, sigma),
+           offsetof(clm_softcore, form));
+    printf("%zu %zu %zu %zu\\n", sizeof(clm_box_info), offsetof(clm_box_info, cutoff), offsetof(clm_box_info, input),
+           offsetof(clm_box_info, origin));
+    return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    sc = [int(v) for v in out[0].split()]
+    S = clm._lib.SoftCore
+    assert sc == [C.sizeof(S), S.eps.offset, S.sigma.offset, S.form.offset]
+    bi = [int(v) for v in out[1].split()]
+    B = clm._lib.BoxInfo
+    assert bi == [C.sizeof(B), B.cutoff.offset, B.input.offset, B.origin.offset]
