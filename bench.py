@@ -39,7 +39,8 @@ def make_workload(n, seed=2):
     M = CELL * (n / 1e5) ** (1.0 / 3.0)
     f = rng.random((n, 3))
     x = (M @ f.T).T
-    return M.astype(np.float32), x.astype(np.float32)
+    # row-major (N, 3), the layout of the reference's vector of 3-vectors
+    return M.astype(np.float32), np.ascontiguousarray(x, dtype=np.float32)
 
 
 def kernel_stats(ctx, name):

@@ -79,7 +79,7 @@ class BatchedSystems:
         if n != self.n or dim != self.dim:
             raise DimensionMismatch("positions must hold every particle of every system")
         if on_dev:
-            self._ctx.use_torch_stream()
+            self._ctx.use_torch_stream(2)
         self._ctx.check(self._ctx.lib.clm_set_positions(self._ctx.ptr, _ptr(xs), int(n), int(on_dev)), "positions")
         if on_dev:
             self._ctx.torch_after()

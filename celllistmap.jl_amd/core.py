@@ -168,22 +168,27 @@ class _Context:
     def check(self, code, what=""):
         _raise(self.ptr, code, what)
 
-    def use_torch_stream(self):
+    def _torch_stream(self) -> int:
+        """Torch's current stream on the context's device, as a raw hipStream_t."""
+        torch = _torch()
+        raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        return raw(self.device) if raw is not None else torch.cuda.current_stream(self.device).cuda_stream
+
+    def use_torch_stream(self, both: int = 1):
         """Order the library's stream after the work pending on torch's current stream
-        (which wrote the tensors the next call reads), and torch's stream after the
-        library's (clm_join_stream: two cached events, recorded and waited in C).  The
-        library keeps its own (non-blocking) stream: it never launches on torch's null
-        stream, which graph capture cannot use."""
-        s = _torch().cuda.current_stream(self.device).cuda_stream
-        self.check(self.lib.clm_join_stream(self.ptr, C.c_void_p(s), 1), "join_stream")
+        (which wrote the tensors the next call reads) and, with both=1, torch's stream
+        after the library's (clm_join_stream: two cached events, recorded and waited in
+        C).  both=2 orders the library's stream only: for calls that end with a host
+        synchronisation or with torch_after().  The library keeps its own (non-blocking)
+        stream: it never launches on torch's null stream, which graph capture cannot use."""
+        self.check(self.lib.clm_join_stream(self.ptr, C.c_void_p(self._torch_stream()), int(both)), "join_stream")
         self._joined = True
 
     def torch_after(self):
         """Make torch's current stream wait for work the library left queued (an
         asynchronous device-to-device copy of a tensor torch may overwrite next)."""
         if getattr(self, "_joined", False):
-            s = _torch().cuda.current_stream(self.device).cuda_stream
-            self.check(self.lib.clm_join_stream(self.ptr, C.c_void_p(s), 0), "join_stream")
+            self.check(self.lib.clm_join_stream(self.ptr, C.c_void_p(self._torch_stream()), 0), "join_stream")
 
 
 def _positions(x, dtype_code: Optional[int], dim: Optional[int] = None):
@@ -453,7 +458,7 @@ class ParticleSystem:
 
     def _set_positions(self, xs, on_dev, n):
         if on_dev:
-            self._ctx.use_torch_stream()
+            self._ctx.use_torch_stream(2)
         self._ctx.check(self._ctx.lib.clm_set_positions(self._ctx.ptr, _ptr(xs), int(n), int(on_dev)), "positions")
         if on_dev:
             self._ctx.torch_after()
@@ -462,7 +467,7 @@ class ParticleSystem:
 
     def _set_ypositions(self, ys, on_dev, n):
         if on_dev:
-            self._ctx.use_torch_stream()
+            self._ctx.use_torch_stream(2)
         self._ctx.check(self._ctx.lib.clm_set_positions_y(self._ctx.ptr, _ptr(ys), int(n), int(on_dev)), "ypositions")
         if on_dev:
             self._ctx.torch_after()
@@ -537,8 +542,8 @@ def pairwise_(op, sys: ParticleSystem, *, reset: bool = True, show_progress: boo
         op = sys.output_op
     ctx = sys._ctx
     L = ctx.lib
-    if device_output:
-        ctx.use_torch_stream()
+    if device_output:  # every operator below ends with a read-back (a host synchronisation)
+        ctx.use_torch_stream(2)
     if isinstance(op, SoftCoreEnergyForces):
         p = op.params()
         _set_types(sys, op.types)
@@ -569,7 +574,7 @@ def pairwise_(op, sys: ParticleSystem, *, reset: bool = True, show_progress: boo
         if n != sys.n or dim != sys.dim:
             raise DimensionMismatch("velocities must be (N, dim)")
         if on_dev:
-            ctx.use_torch_stream()
+            ctx.use_torch_stream(2)
         ctx.check(L.clm_set_velocities(ctx.ptr, _ptr(vs), int(n), int(on_dev)), "velocities")
         if sys.two_set:
             yv = op.yvelocities
@@ -579,7 +584,7 @@ def pairwise_(op, sys: ParticleSystem, *, reset: bool = True, show_progress: boo
             if ny != sys.ny or ydim != sys.dim:
                 raise DimensionMismatch("yvelocities must be (Ny, dim)")
             if y_dev:
-                ctx.use_torch_stream()
+                ctx.use_torch_stream(2)
             ctx.check(L.clm_set_velocities_y(ctx.ptr, _ptr(ys), int(ny), int(y_dev)), "yvelocities")
         if on_dev or (sys.two_set and y_dev):
             ctx.torch_after()
@@ -637,7 +642,7 @@ def _set_types(sys, types):
     if tt.shape[0] != nx + ny:
         raise DimensionMismatch("types must have one entry per particle (x then y for two sets)")
     if on_dev:
-        ctx.use_torch_stream()
+        ctx.use_torch_stream(2)
     ctx.check(ctx.lib.clm_set_types(ctx.ptr, _ptr(tt[:nx]), int(nx), int(on_dev)), "types")
     if ny:
         ctx.check(ctx.lib.clm_set_types_y(ctx.ptr, _ptr(tt[nx:]), int(ny), int(on_dev)), "types")

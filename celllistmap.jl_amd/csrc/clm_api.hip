@@ -1334,12 +1334,15 @@ clm_status clm_join_stream(clm_ctx *c, void *other, int both) {
     if (o == c->stream) return CLM_OK;
     for (hipEvent_t &e : c->join_ev)
         if (!e) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    if (both) {
+    if (both < 0 || both > 2) return fail(c, CLM_ERR_ARG, "join_stream: both must be 0, 1 or 2");
+    if (both >= 1) {
         HIPCHK(c, hipEventRecord(c->join_ev[0], o));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev[0], 0));
     }
-    HIPCHK(c, hipEventRecord(c->join_ev[1], c->stream));
-    HIPCHK(c, hipStreamWaitEvent(o, c->join_ev[1], 0));
+    if (both <= 1) {
+        HIPCHK(c, hipEventRecord(c->join_ev[1], c->stream));
+        HIPCHK(c, hipStreamWaitEvent(o, c->join_ev[1], 0));
+    }
     return CLM_OK;
 }
 

@@ -85,7 +85,8 @@ clm_status clm_set_stream(clm_ctx *ctx, void *stream);
 clm_status clm_get_stream(clm_ctx *ctx, void **stream);
 /* Orders the context's stream and `other` (a hipStream_t passed as void*): with both = 1,
  * work queued on either stream after the call starts after the work queued on both
- * before it; with both = 0 only `other` waits for the context's earlier work.  No
+ * before it; with both = 0 only `other` waits for the context's earlier work; with
+ * both = 2 only the context's stream waits for the work queued on `other`.  No
  * reference counterpart: plumbing for callers whose buffers another stream writes. */
 clm_status clm_join_stream(clm_ctx *ctx, void *other, int both);
 /* Library version string. */

@@ -79,3 +79,39 @@ def test_device_resident_roundtrip(clm, O):
     i, j, d2, d = O.pairs(O.make_box([40.0] * 3, 6.0, dtype=np.float32), x, 1)
     ko, order = O.canonical_pairs(i, j)
     assert np.array_equal(kg, ko) and np.array_equal(dg.astype(np.float64), d[order])
+
+
+def test_side_stream_ordering(clm, O):
+    """Positions written on a caller's side stream right before update_, and overwritten
+    there right after it: the library reads what was written before the call and nothing
+    written after it (joins with torch's current stream, both directions)."""
+    import ctypes as C
+    import torch
+    say("[streams] side stream")
+    rng = np.random.default_rng(17)
+    L, rc = 40.0, 6.0
+    xa = (rng.random((6000, 3)) * L).astype(np.float32)
+    xb = (rng.random((6000, 3)) * L).astype(np.float32)
+    box = O.make_box([L] * 3, rc, dtype=np.float32)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        xt = torch.from_numpy(xb).cuda()
+        nl = clm.InPlaceNeighborList(xpositions=xt, cutoff=rc, unitcell=[L] * 3, device_output=True)
+        busy = torch.ones(32 << 20, device="cuda:0")
+        for step, (new, nxt) in enumerate(((xa, xb), (xb, xa))):
+            for _ in range(8):  # keep the side stream busy before the write the library must see
+                busy.mul_(1.0)
+            xt.copy_(torch.from_numpy(new).to("cuda:0", non_blocking=False))
+            clm.update_(nl, xpositions=xt)
+            xt.copy_(torch.from_numpy(nxt).cuda())  # after the call: must not be seen
+            res = clm.neighborlist_(nl)
+            kg, dg = canon(res)
+            i, j, d2, d = O.pairs(box, new, 1)
+            ko, order = O.canonical_pairs(i, j)
+            assert np.array_equal(kg, ko), step
+            assert np.array_equal(dg.astype(np.float64), d[order]), step
+    torch.cuda.synchronize()
+    ctx = nl.sys._ctx
+    raw = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert ctx.lib.clm_join_stream(ctx.ptr, raw, 2) == 0
+    assert ctx.lib.clm_join_stream(ctx.ptr, raw, 3) == 1  # CLM_ERR_ARG
