@@ -623,7 +623,7 @@ clm_status bin_bucketed(clm_ctx *c, bool &done) {
         ScanState st;
         if ((s = scan_state(c, SCAN_CELLS, c->ncells, st))) return s;
         ProdCells pc{(const unsigned long long *)c->cnt64.p, (int *)c->cnt.p, (int *)c->creal.p};
-        LAUNCH(c, "k_scan_cells", hipLaunchKernelGGL((k_scan_pair<ProdCells>), dim3((unsigned)scan_tiles(c->ncells)),
+        LAUNCH(c, "k_scan_cells", hipLaunchKernelGGL((k_scan_pair<ProdCells>), dim3((unsigned)scan_tiles_for<ProdCells>(c->ncells)),
                                                      dim3(SCAN_THREADS), 0, c->stream, (long long)c->ncells, pc,
                                                      (const unsigned long long *)nullptr, (int *)c->cstart.p, (int *)c->rstart.p, (int *)c->flags.p + 1,
                                                      (int *)c->flags.p + 2, st, (unsigned int *)c->flags.p, FLAG_SCAN));
@@ -771,7 +771,7 @@ clm_status enqueue_mode(clm_ctx *c, int mode) {
     ScanState st;
     if ((s = scan_state(c, SCAN_ROWS, c->nrows, st))) return s;
     ProdRows<T> pr{(const SysDesc<T> *)c->sys_desc.p, (const int *)c->rowoff.p, home_start(c, mode), c->nsys};
-    LAUNCH(c, "k_scan_rows", hipLaunchKernelGGL((k_scan_pair<ProdRows<T>>), dim3((unsigned)scan_tiles(c->nrows)),
+    LAUNCH(c, "k_scan_rows", hipLaunchKernelGGL((k_scan_pair<ProdRows<T>>), dim3((unsigned)scan_tiles_for<ProdRows<T>>(c->nrows)),
                                                 dim3(SCAN_THREADS), 0, c->stream, (long long)c->nrows, pr,
                                                 (const unsigned long long *)nullptr, (int *)c->tstart[mode].p, (int *)nullptr, (int *)nullptr, (int *)nullptr, st,
                                                 (unsigned int *)c->flags.p, FLAG_SCAN));
@@ -964,7 +964,7 @@ clm_status neighborlist_t(clm_ctx *c, int32_t *oi, int32_t *oj, void *od, int64_
         ScanState st;
         if ((s = scan_state(c, SCAN_CHUNKS, nchunks, st))) return s;
         ProdHoles ph{(const int *)c->ch_used.p, (long long)cap};
-        LAUNCH(c, "k_scan_holes", hipLaunchKernelGGL((k_scan_pair<ProdHoles>), dim3((unsigned)scan_tiles(nchunks)),
+        LAUNCH(c, "k_scan_holes", hipLaunchKernelGGL((k_scan_pair<ProdHoles>), dim3((unsigned)scan_tiles_for<ProdHoles>(nchunks)),
                                                      dim3(SCAN_THREADS), 0, c->stream, (long long)nchunks, ph, tot,
                                                      (int *)c->ch_hstart.p, (int *)c->ch_mstart.p, (int *)nullptr,
                                                      (int *)nullptr, st, ovf, 2u));

@@ -436,7 +436,7 @@ __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
 // ------------------------------------------------------------------------------
 //
 // Exclusive scans of two counts per element, produced on the fly by Prod::get(i, a, b),
-// in one launch: each workgroup takes a tile of SCAN_TILE elements in the order it
+// in one launch: each workgroup takes a tile of SCAN_THREADS * Prod::ITEMS elements in the order it
 // started (a ticket counter), publishes the tile's sums, then walks back over the
 // preceding tiles' published words until it meets a tile that published its inclusive
 // prefix ("decoupled look-back").  A tile only waits on tiles that started before it and
@@ -456,7 +456,11 @@ struct ScanState {
     unsigned int *done;
 };
 
+// Tiles of a scan over n elements (the most any producer needs: it sizes the scan state);
+// a producer with Prod::ITEMS elements per thread launches scan_tiles_for<Prod>(n).
 inline long long scan_tiles(long long n) { return n / SCAN_TILE + 1; }
+template <typename Prod>
+inline long long scan_tiles_for(long long n) { return n / ((long long)SCAN_THREADS * Prod::ITEMS) + 1; }
 
 __device__ inline unsigned long long scan_pack(int a, int b) {
     return ((unsigned long long)(unsigned)a << 31) | (unsigned long long)(unsigned)b;
@@ -480,11 +484,13 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_pair(long long n, Prod pr
     __syncthreads();
     const int tile = s_tile;
     const unsigned long long u = s_u;
-    const long long base = (long long)tile * SCAN_TILE + (long long)t * SCAN_ITEMS;
-    int va[SCAN_ITEMS], vb[SCAN_ITEMS];
+    constexpr int IT = Prod::ITEMS;
+    static_assert(IT >= SCAN_ITEMS, "scan_tiles sizes the state for SCAN_ITEMS per thread");
+    const long long base = (long long)tile * (SCAN_THREADS * IT) + (long long)t * IT;
+    int va[IT], vb[IT];
     int sa = 0, sb = 0, mx = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
+    for (int k = 0; k < IT; ++k) {
         int a = 0, b = 0;
         if (base + k < n) prod.get(base + k, u, a, b);
         va[k] = a;
@@ -558,7 +564,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_pair(long long n, Prod pr
     __syncthreads();
     int ea = s_pa + wa + ia - sa, eb = s_pb + wb + ib - sb;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
+    for (int k = 0; k < IT; ++k) {
         const long long i = base + k;
         if (i <= n) {
             if (out_a) out_a[i] = ea;
@@ -584,6 +590,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_pair(long long n, Prod pr
 // Per-cell counts of the single-pass binning: entries in the low and real particles in
 // the high half of each 64-bit counter (also written out as cnt / creal).
 struct ProdCells {
+    static constexpr int ITEMS = SCAN_ITEMS;
     const unsigned long long *cnt64;
     int *cnt, *creal;
     __device__ void get(long long i, unsigned long long, int &a, int &b) const {
@@ -598,6 +605,7 @@ struct ProdCells {
 // Wave tasks of every row of cells: ceil(homes of the row / 64).
 template <typename T>
 struct ProdRows {
+    static constexpr int ITEMS = SCAN_ITEMS;
     const SysDesc<T> *sys;
     const int *rowoff;
     const int *hstart;
@@ -1303,6 +1311,7 @@ struct OpNeighborList {
 // Per chunk of the neighbour-list output: holes below the pair count P (the uniform
 // value) and entries at or beyond P (movers); none when P exceeds the capacity.
 struct ProdHoles {
+    static constexpr int ITEMS = 32;  // ~1M chunks: fewer, larger tiles shorten the look-back
     const int *used;
     long long cap;
     __device__ void get(long long c, unsigned long long total, int &a, int &b) const {
