@@ -1335,18 +1335,45 @@ __device__ inline long long last_le64(const int *__restrict__ a, long long lo, l
     return lo;
 }
 
+// The chunk of element k of a scan st (non-decreasing, st[nchunks] = the total) for the
+// wave's elements [kw, klast] (kw, klast wave-uniform, kw <= k <= klast < st[nchunks]):
+// one wave-uniform bisection for kw and one more per chunk end inside the range, in
+// place of a bisection per lane.
+__device__ inline long long chunk_of_wave(const int *__restrict__ st, long long nchunks, long long kw, long long k,
+                                          long long klast) {
+    long long c = (long long)uniform_u64((unsigned long long)last_le64(st, 0, nchunks, kw));
+    long long mine = c;
+    while (c + 1 < nchunks) {  // c rises on every pass
+        const long long nxt = st[c + 1];  // the first element past chunk c
+        if (nxt > klast) break;
+        c = (long long)uniform_u64((unsigned long long)last_le64(st, c + 1, nchunks, nxt));
+        if (k >= nxt) mine = c;
+    }
+    return mine;
+}
+
 // Moves the entries at or beyond the pair count P (the movers) into the holes below it:
-// hole k (in the order of hstart) takes mover k (in the order of mstart).
+// hole k (in the order of hstart) takes mover k (in the order of mstart).  Each wave
+// moves 64 consecutive entries per step and finds their chunks with wave-uniform
+// searches.
 template <typename T>
-__global__ void k_nl_fill(long long nchunks, const int *__restrict__ used, const int *__restrict__ hstart,
-                          const int *__restrict__ mstart, const unsigned long long *__restrict__ total, NLOut<T> out) {
+__global__ void __launch_bounds__(256) k_nl_fill(long long nchunks, const int *__restrict__ used,
+                                                 const int *__restrict__ hstart, const int *__restrict__ mstart,
+                                                 const unsigned long long *__restrict__ total, NLOut<T> out) {
     const long long H = hstart[nchunks];
     const long long P = (long long)*total;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < H; k += (long long)gridDim.x * blockDim.x) {
-        const long long cm = last_le64(mstart, 0, nchunks, k);
+    const int lane = threadIdx.x & 63;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long kw = (long long)uniform_u64((unsigned long long)((long long)blockIdx.x * blockDim.x + (threadIdx.x - lane)));
+         kw < H; kw += stride) {
+        const long long k = kw + lane;
+        const long long klast = min(kw + 63, H - 1);
+        const long long cm = chunk_of_wave(mstart, nchunks, kw, k, klast);
+        const long long ch = chunk_of_wave(hstart, nchunks, kw, k, klast);
+        if (k >= H) continue;
         const long long src = max(cm * NL_CHUNK, P) + (k - mstart[cm]);
-        const long long ch = last_le64(hstart, 0, nchunks, k);
         const long long dst = ch * NL_CHUNK + used[ch] + (k - hstart[ch]);
+        if (src < 0 || dst < 0 || src >= out.cap + out.scap) continue;  // only after a failed scan
         int i, j;
         T d;
         out.get(src, i, j, d);
