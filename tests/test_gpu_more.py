@@ -115,3 +115,56 @@ def test_side_stream_ordering(clm, O):
     raw = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     assert ctx.lib.clm_join_stream(ctx.ptr, raw, 2) == 0
     assert ctx.lib.clm_join_stream(ctx.ptr, raw, 3) == 1  # CLM_ERR_ARG
+
+
+@pytest.mark.parametrize("tric", [False, True])
+def test_capacity_boundaries(clm, O, tric):
+    """clm_neighborlist at capacities P - 1, P, P + 1 and P + 37 (P = the pair count),
+    host and device output: below P it reports CLM_ERR_CAPACITY with the required count;
+    from P on the compacted list is exactly the oracle's."""
+    import ctypes as C
+    import torch
+    _lib = clm._lib
+    say(f"[capacity] boundaries tric={tric}")
+    rng = np.random.default_rng(31)
+    if tric:
+        uc = np.array([[40.0, 8.0, 4.0], [0.0, 40.0, 6.0], [0.0, 0.0, 40.0]])
+        x = (uc @ rng.random((3, 9000))).T.astype(np.float32)
+    else:
+        uc = np.array([40.0, 40.0, 40.0])
+        x = (rng.random((9000, 3)) * uc).astype(np.float32)
+    rc = 4.0
+    sys_ = clm.ParticleSystem(xpositions=x, unitcell=uc, cutoff=rc, dtype=np.float32)
+    box = O.make_box(uc.astype(np.float32) if tric else uc, rc, dtype=np.float32)
+    i, j, d2, d = O.pairs(box, x, 1)
+    ko, order = O.canonical_pairs(i, j)
+    P = len(ko)
+    assert P > 1000
+    ctx = sys_._ctx
+    for on_dev in (0, 1):
+        for cap in (P - 1, P, P + 1, P + 37):
+            if on_dev:
+                oi = torch.empty(cap, dtype=torch.int32, device="cuda:0")
+                oj = torch.empty(cap, dtype=torch.int32, device="cuda:0")
+                od = torch.empty(cap, dtype=torch.float32, device="cuda:0")
+                ptrs = [C.c_void_p(t.data_ptr()) for t in (oi, oj, od)]
+            else:
+                oi = np.empty(cap, np.int32)
+                oj = np.empty(cap, np.int32)
+                od = np.empty(cap, np.float32)
+                ptrs = [t.ctypes.data_as(C.c_void_p) for t in (oi, oj, od)]
+            n = C.c_int64(0)
+            if on_dev:
+                ctx.use_torch_stream(2)
+            rc_ = ctx.lib.clm_neighborlist(ctx.ptr, *ptrs, cap, C.byref(n), on_dev)
+            if cap < P:
+                assert rc_ == _lib.CLM_ERR_CAPACITY and n.value == P, (on_dev, cap, rc_, n.value)
+                continue
+            assert rc_ == 0 and n.value == P, (on_dev, cap, rc_, n.value)
+            if on_dev:
+                gi, gj, gd = (t[:P].cpu().numpy() for t in (oi, oj, od))
+            else:
+                gi, gj, gd = oi[:P], oj[:P], od[:P]
+            kg, og = O.canonical_pairs(gi.astype(np.int64), gj.astype(np.int64))
+            assert np.array_equal(kg, ko), (on_dev, cap)
+            assert np.array_equal(gd[og].astype(np.float64), d[order]), (on_dev, cap)
