@@ -317,11 +317,25 @@ class NeighborListResult:
     """Neighbour list (i, j, d) arrays; iterates as tuples like the reference's
     Vector{Tuple{Int,Int,T}} (indices 0-based here)."""
 
-    def __init__(self, i, j, d):
-        self.i, self.j, self.d = i, j, d
+    def __init__(self, i, j, d, n=None):
+        # i, j, d may be longer than the list (the reused output buffers): the first n
+        # entries are the list, taken as views on first access
+        self._full = (i, j, d)
+        self._n = int(i.shape[0]) if n is None else int(n)
+        self._views = None
+
+    def _view(self, k):
+        if self._views is None:
+            n = self._n
+            self._views = tuple(a if a.shape[0] == n else a[:n] for a in self._full)
+        return self._views[k]
+
+    i = property(lambda self: self._view(0))
+    j = property(lambda self: self._view(1))
+    d = property(lambda self: self._view(2))
 
     def __len__(self):
-        return int(self.i.shape[0])
+        return self._n
 
     def __iter__(self):
         for a, b, c in zip(np.asarray(self.i.cpu() if _is_torch(self.i) else self.i).tolist(),
@@ -712,7 +726,7 @@ def _neighborlist(sys: ParticleSystem, capacity, device_output: bool):
         ctx.check(rc, "neighborlist!")
         n = npairs.value
         sys._nl_capacity = capacity
-        return NeighborListResult(i[:n], j[:n], d[:n])
+        return NeighborListResult(i, j, d, n)
 
 
 # ---------------------------------------------------------------------------
