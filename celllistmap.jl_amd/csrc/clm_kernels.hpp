@@ -1179,7 +1179,10 @@ template <typename T>
 struct OpNeighborList {
     static constexpr int S = 512;
     static constexpr int CH = NL_CHUNK;
-    static constexpr int PREFETCH = 384;  // request the next chunk when fewer slots remain
+    // request the next chunk when fewer slots remain: a request left unused when the wave
+    // ends leaves a chunk of holes for k_nl_fill, so the request is made as late as still
+    // hides its latency (256 measured best of 128, 192, 256, 384, 480)
+    static constexpr int PREFETCH = 256;
     struct Shared {
         int i[S];
         int j[S];
