@@ -1,4 +1,4 @@
-"""The bench configuration at full size: N = 1,000,000 float32 particles uniform in the
+"""The bench configuration at full size: N = 1,000,000 particles (float32, and float64) uniform in the
 triclinic cell [[100,20,10],[0,100,15],[0,0,100]] * 10^(1/3) (lattice vectors as
 columns), cutoff 12 -- about 3.6e8 pairs, more than the oracle's pair list can hold in
 a test.  Checked through properties that hold at any size: the pair count against the
@@ -16,17 +16,18 @@ def say(*a):
     print(*a, flush=True)
 
 
-def test_bench_config_full_size(clm):
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_bench_config_full_size(clm, dtype):
     import torch
-    say("[fullsize] N=1e6 f32 triclinic, two sweep kernels")
+    say(f"[fullsize] N=1e6 {np.dtype(dtype).name} triclinic, two sweep kernels")
     N, rc = 1_000_000, 12.0
     rng = np.random.default_rng(2)
     M = np.array([[100.0, 20.0, 10.0], [0.0, 100.0, 15.0], [0.0, 0.0, 100.0]]) * 10.0 ** (1.0 / 3.0)
-    x = (M @ rng.random((3, N))).T.astype(np.float32)
+    x = np.ascontiguousarray((M @ rng.random((3, N))).T.astype(dtype))
     xt = torch.from_numpy(x).to("cuda:0")
     sorted_lists = {}
     for variant in (1, 0):
-        nl = clm.InPlaceNeighborList(xpositions=xt, cutoff=rc, unitcell=M.astype(np.float32), lcell=2,
+        nl = clm.InPlaceNeighborList(xpositions=xt, cutoff=rc, unitcell=M.astype(dtype), lcell=2,
                                      device_output=True)
         nl.sys.set_sweep_variant(variant)
         r = clm.neighborlist_(nl)
