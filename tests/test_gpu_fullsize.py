@@ -48,3 +48,28 @@ def test_bench_config_full_size(clm, dtype):
     assert float(d1.min()) > 0.0 and float(d1.max()) <= rc
     assert torch.equal(k1, k0)
     assert torch.equal(d1, d0)
+
+
+def test_bench_config_energy_forces_full_size(clm):
+    """Soft-core energy and forces of the same full-size system at float32 and float64:
+    within 1e-4 of each other (the float32 tolerance), and the float64 forces sum to zero
+    (Newton's third law in a periodic box)."""
+    import torch
+    say("[fullsize] N=1e6 energy/forces, f32 against f64")
+    N, rc = 1_000_000, 12.0
+    rng = np.random.default_rng(2)
+    M = np.array([[100.0, 20.0, 10.0], [0.0, 100.0, 15.0], [0.0, 0.0, 100.0]]) * 10.0 ** (1.0 / 3.0)
+    x64 = np.ascontiguousarray((M @ rng.random((3, N))).T)
+    out = {}
+    for dtype in (np.float64, np.float32):
+        xt = torch.from_numpy(x64.astype(dtype)).to("cuda:0")
+        sys_ = clm.ParticleSystem(xpositions=xt, unitcell=M.astype(dtype), cutoff=rc, lcell=2)
+        r = clm.pairwise_(clm.SoftCoreEnergyForces(0.2, 3.0), sys_, device_output=True)
+        out[dtype] = (r.energy, r.forces.double())
+        del sys_, xt
+    e64, f64 = out[np.float64]
+    e32, f32 = out[np.float32]
+    say(f"     energy f64 {e64:.10g} f32 {e32:.10g}")
+    assert abs(e32 - e64) <= 1e-4 * abs(e64)
+    assert float(torch.linalg.norm(f32 - f64)) <= 1e-4 * float(torch.linalg.norm(f64))
+    assert float(torch.linalg.norm(f64.sum(0))) <= 1e-8 * float(f64.abs().sum())
